@@ -1,0 +1,142 @@
+/* moao_hip.h -- C-ABI of the MI355X (gfx950) MOAO reconstructor path.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every matrix is column-major with an
+ * explicit leading dimension, exactly as in the reference.  Two flavours of each compute entry:
+ *   - host-pointer entries  (moao_hip_<op>)      : drop-in for the reference's CPU functions;
+ *     they stage through device memory internally and return when the result is back on the
+ *     host, with the reference's in-place semantics;
+ *   - device-pointer entries (moao_hip_<op>_dev) : operate on device buffers, stream-ordered on
+ *     `stream` (NULL = the library stream).
+ * Status: 0 = ok; >0 = LAPACK-style info (e.g. the order of the first non-positive pivot);
+ * <0 = one of the MOAO_ERR_* codes below.  moao_last_error() returns a message for the
+ * calling thread's last failure.
+ */
+#ifndef MOAO_HIP_H
+#define MOAO_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOAO_OK 0
+#define MOAO_ERR_ARG (-1)      /* invalid argument (the reference exits or misbehaves) */
+#define MOAO_ERR_HIP (-2)      /* HIP runtime error */
+#define MOAO_ERR_NOMEM (-3)    /* device allocation failed / over the HBM budget */
+#define MOAO_ERR_NOCONV (-4)   /* iterative refinement did not reach its tolerance */
+#define MOAO_ERR_IO (-5)       /* parameter file missing, truncated or inconsistent */
+#define MOAO_ERR_NODEV (-6)    /* no usable gfx950 device */
+
+/* Precision modes of the reconstructor. */
+#define MOAO_FP64 0            /* FP64 everywhere: the reference's lapack arithmetic */
+#define MOAO_MIXED_FP32 1      /* FP32 trailing updates, FP64 diagonal tiles + FP64 refinement */
+
+const char* moao_last_error(void);
+const char* moao_version(void);
+
+/* ---------------------------------------------------------------- system model (host) ----
+ * Replaces Tomo_struct<double> (common/tomo_struct/tomo_struct.hpp:9-94): reads the
+ * reference's sys-params / atm-params text files and prepares the vectors matcov reads.   */
+typedef struct moao_tomo moao_tomo;
+
+/* POD view of what the matcov generator reads; mirrors the argument list of the reference's
+ * matcov_tile_gpu_4 (common/kernels/matcov_kernels_gpu.hpp:15-20) plus the truth-sensor
+ * X, Y used by Ctt (matcov_ts_kernel_tile, common/kernels/matcov_kernels.cpp:495-515).
+ * All pointers are HOST pointers.  Callers holding their own Tomo_struct fill it from its
+ * fields (see moao_hip.hpp). */
+typedef struct moao_tomo_view {
+    int Nw;                 /* number of WFS including the truth sensor (last) */
+    int Nlayer;             /* turbulent layers */
+    int nlgs;               /* LGS count (the first nlgs WFS) */
+    long Nx;                /* valid subapertures over all WFS */
+    double diam;            /* telescope diameter (m) */
+    double lgs_cst;         /* constant added to LGS xx/yy blocks */
+    const long* Nsubap;     /* [Nw] */
+    const long* Nssp;       /* [Nw] */
+    const double* sspSizeL; /* [Nw*Nlayer], w-major */
+    const double* u;        /* [Nlayer*Nx], layer-major */
+    const double* v;        /* [Nlayer*Nx] */
+    const double* X;        /* [Nx] unprojected subaperture centres */
+    const double* Y;        /* [Nx] */
+    const double* L0;       /* [Nlayer] outer scale per layer (the reference's L0diff) */
+    const double* cn2;      /* [Nlayer] */
+    const double* noiseNGS; /* [Nw-nlgs] */
+    const double* noiseLGSxx; /* [Nx] */
+    const double* noiseLGSyy; /* [Nx] */
+    const double* noiseLGSxy; /* [Nx] */
+} moao_tomo_view;
+
+/* Tomo_struct(sys_file, atm_file)  (tomo_struct.cpp:11-17) */
+int moao_tomo_create(const char* sys_file, const char* atm_file, moao_tomo** out);
+/* Tomo_struct::updateAtm(atm_file)  (tomo_struct.cpp:212-262) */
+int moao_tomo_update_atm(moao_tomo* t, const char* atm_file);
+/* Tomo_struct::setTarget(i)  (tomo_struct.cpp:275-279) */
+int moao_tomo_set_target(moao_tomo* t, int target);
+/* Tomo_struct::setTargetCoord(aX, aY, toRad)  (tomo_struct.cpp:291-303) */
+int moao_tomo_set_target_coord(moao_tomo* t, double ax, double ay, int to_rad);
+/* Fills *view with pointers into t (valid until the next update/set call or destroy). */
+int moao_tomo_get_view(const moao_tomo* t, moao_tomo_view* view);
+/* getNMeas / getNMeasTS  (tomo_struct.cpp:569-579) */
+long moao_tomo_nmeas(const moao_tomo* t);
+long moao_tomo_nmeasts(const moao_tomo* t);
+/* nW, nLgs, nNgs, nTarget, nLayer, Nssp: a small int summary (out[6]) */
+int moao_tomo_dims(const moao_tomo* t, long* out6);
+/* Copies a named vector field ("X","Y","u","v","sspSizeL","L0","cn2","noiseNGS",
+ * "noiseLGSxx","noiseLGSyy","noiseLGSxy","lgsExt","lgsTheta","Nsubap","Nssp") into out
+ * (as double); returns its length, or MOAO_ERR_ARG if the name is unknown.  out may be NULL
+ * to query the length. */
+long moao_tomo_field(const moao_tomo* t, const char* name, double* out, long cap);
+void moao_tomo_destroy(moao_tomo* t);
+
+/* ---------------------------------------------------------------- device / memory --------- */
+/* Selects and initialises the device (one process, one GPU).  Called implicitly with device 0
+ * by the first compute entry if the caller did not. */
+int moao_hip_init(int device);
+int moao_hip_finalize(void);
+int moao_hip_malloc(void** dptr, size_t bytes);
+int moao_hip_free(void* dptr);
+int moao_hip_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int moao_hip_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int moao_hip_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int moao_hip_synchronize(void);
+/* Device memory in use / total, bytes. */
+int moao_hip_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ---------------------------------------------------------------- matcov ------------------
+ * MOAO_COMMON::matcov_tile (common/compute/matcov.hpp:14-18, matcov.cpp:17-98).
+ * type_mat: 1 = Cmm (only global row >= global col is written), 2 = Cmaa (full, reference
+ * quirk kept), 3 = Ctm (full), 4 = Ctt (full).  xoffset is the global row offset and yoffset
+ * the global column offset of the tile; element (lx, ly) goes to data[ly*lda + lx].
+ * Unknown type_mat returns MOAO_ERR_ARG (the reference calls exit(1)).                    */
+int moao_hip_matcov_tile(double* data, int nrows, int ncols, int xoffset, int yoffset,
+                         int lda, const moao_tomo_view* tomo, int type_mat);
+int moao_hip_matcov_tile_dev(double* d_data, int nrows, int ncols, int xoffset, int yoffset,
+                             int lda, const moao_tomo_view* tomo, int type_mat, void* stream);
+
+/* ---------------------------------------------------------------- reconstructor -----------
+ * MOAO_LAPACK::reconstructor (lapack/compute/reconstructor.hpp:19-21, reconstructor.cpp:32-58):
+ * dpotrf('L', Cmm) then, per target g, Ctm_g = Ctm + g*nmeas*nmeasts is overwritten by
+ * R_g = Ctm_g * Cmm^{-1} (dtrsm Right/Lower/Trans then Right/Lower/NoTrans), FP64.
+ * Returns 0, the dpotrf info (> 0: leading minor of that order not positive definite; Cmm is
+ * left partially factored and, for the host entry, Ctm untouched), or MOAO_ERR_*.            */
+int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, double* Ctm,
+                           long ldCtm, long ngal);
+/* Device buffers; on info > 0 the Ctm blocks hold partial results. */
+int moao_hip_reconstructor_dev(long nmeas, long nmeasts, double* dCmm, long ldCmm, double* dCtm,
+                               long ldCtm, long ngal, void* stream);
+
+/* ---------------------------------------------------------------- GEMM utility ------------
+ * C = alpha*A*op(B) + beta*C on the MFMA engine the stages use (column-major);
+ * op(B) = B^T when transb != 0 (B is N x K), B otherwise (B is K x N).  No reference
+ * counterpart: exposed for tests and for callers composing their own chains.                */
+int moao_hip_dgemm_dev(int transb, int M, int N, int K, double alpha, const double* A, long lda,
+                       const double* B, long ldb, double beta, double* C, long ldc, void* stream);
+int moao_hip_sgemm_dev(int transb, int M, int N, int K, float alpha, const float* A, long lda,
+                       const float* B, long ldb, float beta, float* C, long ldc, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOAO_HIP_H */

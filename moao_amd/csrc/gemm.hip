@@ -1,0 +1,215 @@
+// MFMA GEMM for gfx950 (FP64 via v_mfma_f64_16x16x4f64, FP32 via v_mfma_f32_16x16x4f32).
+//
+// Workgroup: 256 lanes = 4 waves in a 2x2 arrangement over a 128x128 C tile; each wave owns a
+// 64x64 sub-tile = 4x4 MFMA blocks of 16x16 held in accumulator registers.  K advances in
+// slices of 16 through double-buffered LDS (k-major, m/n contiguous, padded rows), with the
+// next slice's global loads issued before the current slice's MFMAs so one barrier per slice
+// suffices.  MFMA operand layout (checked on the device): lane l supplies A[l%16][k0+l/16] and
+// B[k0+l/16][l%16]; accumulator item t of lane l is C[r][l%16] with r = 4*(l/16)+t for f32
+// and r = l/16 + 4t for f64.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace moao {
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 16;
+constexpr int PAD = 2;
+constexpr int NT = 256;
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // accumulator item t of lane l holds row (l/16) + 4t of the 16x16 block
+    static __device__ __forceinline__ int row(int lane, int t) { return (lane >> 4) + 4 * t; }
+};
+
+template <>
+struct Mfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // accumulator item t of lane l holds row 4*(l/16) + t of the 16x16 block
+    static __device__ __forceinline__ int row(int lane, int t) { return 4 * (lane >> 4) + t; }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(NT) gemm_kernel(GemmArgs<T> g) {
+    using acc_t = typename Mfma<T>::acc_t;
+    __shared__ T As[2][BK][BM + PAD];
+    __shared__ T Bs[2][BK][BN + PAD];
+
+    const GemmSeg<T> s = g.seg[blockIdx.z];
+    const int m0 = blockIdx.x * BM;
+    const int n0 = blockIdx.y * BN;
+    if (m0 >= s.M) return;
+    const bool tri = g.tri && blockIdx.z == 0;
+    if (tri && m0 + BM - 1 + g.tri_off < n0) return;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int K = g.K, N = g.N, M = s.M;
+
+    acc_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    // Global -> register staging.  A (and B^T): lane runs down 128 contiguous rows of a column.
+    T ra[8], rb[8];
+    const int lrow = tid & 127, lk = tid >> 7;          // A / B^T pattern: 2 columns per pass
+    const int bk = tid & 15, bn = tid >> 4;             // B (K x N) pattern: 16 k per column
+
+    auto load = [&](int k0) {
+        const int row = m0 + lrow;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = k0 + lk + 2 * r;
+            ra[r] = (row < M && k < K) ? s.A[row + (long)k * s.lda] : T(0);
+        }
+        if (g.transb) {
+            const int col = n0 + lrow;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int k = k0 + lk + 2 * r;
+                rb[r] = (col < N && k < K) ? g.B[col + (long)k * g.ldb] : T(0);
+            }
+        } else {
+            const int k = k0 + bk;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int col = n0 + bn + 16 * r;
+                rb[r] = (col < N && k < K) ? g.B[k + (long)col * g.ldb] : T(0);
+            }
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) As[buf][lk + 2 * r][lrow] = ra[r];
+        if (g.transb) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) Bs[buf][lk + 2 * r][lrow] = rb[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) Bs[buf][bk][bn + 16 * r] = rb[r];
+        }
+    };
+
+    const int nk = (K + BK - 1) / BK;
+    load(0);
+    store(0);
+    __syncthreads();
+    int cur = 0;
+    const int ar = wm * 64 + (lane & 15);
+    const int bc = wn * 64 + (lane & 15);
+    const int kq = lane >> 4;
+    for (int kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            const int k = kk * 4 + kq;
+            T a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[cur][k][ar + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[cur][k][bc + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mac(a[i], b[j], acc[i][j]);
+        }
+        if (kt + 1 < nk) store(cur ^ 1);
+        __syncthreads();
+        cur ^= 1;
+    }
+
+    // Epilogue: C = alpha*acc + beta*C (C not read when beta == 0).
+    const T alpha = g.alpha, beta = g.beta;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+            if (col >= N) continue;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int row = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, t);
+                if (row >= M) continue;
+                if (tri && row + g.tri_off < col) continue;
+                T* c = s.C + row + (long)col * s.ldc;
+                T v = alpha * acc[i][j][t];
+                if (beta != T(0)) v += beta * *c;
+                *c = v;
+            }
+        }
+    }
+}
+
+template <typename T>
+int launch(const GemmArgs<T>& g, hipStream_t stream) {
+    if (g.nseg == 0) return MOAO_OK;
+    if (g.nseg < 0 || g.nseg > kMaxSeg || g.N < 0 || g.K < 0) {
+        set_error("gemm: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    int maxM = 0;
+    for (int i = 0; i < g.nseg; ++i) maxM = g.seg[i].M > maxM ? g.seg[i].M : maxM;
+    if (maxM == 0 || g.N == 0) return MOAO_OK;
+    dim3 grid((maxM + BM - 1) / BM, (g.N + BN - 1) / BN, g.nseg);
+    if (grid.y > 65535) {
+        set_error("gemm: N too large for one launch");
+        return MOAO_ERR_ARG;
+    }
+    hipLaunchKernelGGL(gemm_kernel<T>, grid, dim3(NT), 0, stream, g);
+    MOAO_HIP_TRY(hipGetLastError());
+    return MOAO_OK;
+}
+
+}  // namespace
+
+int gemm_f64(const GemmArgs<double>& a, hipStream_t stream) { return launch(a, stream); }
+int gemm_f32(const GemmArgs<float>& a, hipStream_t stream) { return launch(a, stream); }
+
+}  // namespace moao
+
+extern "C" {
+
+// Plain device GEMM entry (utility; the stages above use the engine directly):
+// C = alpha*A*op(B) + beta*C, op(B) = B^T (transb=1, B is N x K) or B (transb=0, B is K x N).
+int moao_hip_dgemm_dev(int transb, int M, int N, int K, double alpha, const double* A, long lda,
+                       const double* B, long ldb, double beta, double* C, long ldc, void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldc < std::max(1, M) ||
+        ldb < std::max(1, transb ? N : K)) {
+        moao::set_error("dgemm: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    auto a = moao::gemm_args<double>(M, N, K, alpha, A, lda, B, ldb, transb, beta, C, ldc);
+    return moao::gemm_f64(a, moao::pick_stream(stream));
+}
+
+int moao_hip_sgemm_dev(int transb, int M, int N, int K, float alpha, const float* A, long lda,
+                       const float* B, long ldb, float beta, float* C, long ldc, void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldc < std::max(1, M) ||
+        ldb < std::max(1, transb ? N : K)) {
+        moao::set_error("sgemm: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    auto a = moao::gemm_args<float>(M, N, K, alpha, A, lda, B, ldb, transb, beta, C, ldc);
+    return moao::gemm_f32(a, moao::pick_stream(stream));
+}
+
+}  // extern "C"

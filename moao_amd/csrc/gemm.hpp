@@ -1,0 +1,67 @@
+// MFMA GEMM engine shared by the reconstructor and Cee/Cvv stages.
+//
+//   C_g[M_g x N] = alpha * A_g[M_g x K] * op(B) + beta * C_g       for each segment g
+//
+// Column-major, explicit leading dimensions.  op(B) = B^T with B stored N x K (transb = 1) or
+// op(B) = B with B stored K x N (transb = 0).  Up to kMaxSeg row segments share one B and one
+// launch (grid.z = segment): the reconstructor uses this to update the Cmm rows and the Ctm
+// rows of every target with a single launch.  Segment 0 may be restricted to its lower
+// trapezoid (row + tri_off >= col): tiles wholly above that line are skipped and elements
+// above it are not written.
+//
+// In-place use: when C_g == A_g and N <= the kernel's N tile (one column tile), each workgroup
+// reads its whole row block of A before it writes C, and no other workgroup touches those rows,
+// so C := A * op(B) in place is well defined (used for the triangular-inverse panel solves).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace moao {
+
+constexpr int kMaxSeg = 4;
+
+template <typename T>
+struct GemmSeg {
+    const T* A;
+    T* C;
+    long lda, ldc;
+    int M;
+};
+
+template <typename T>
+struct GemmArgs {
+    GemmSeg<T> seg[kMaxSeg];
+    int nseg;
+    const T* B;
+    long ldb;
+    int N, K;
+    int transb;
+    int tri;      // segment 0 lower-trapezoid restriction
+    int tri_off;
+    T alpha, beta;
+};
+
+// Launches on `stream`; returns a MOAO_* status (argument checks only, launch is async).
+int gemm_f64(const GemmArgs<double>& a, hipStream_t stream);
+int gemm_f32(const GemmArgs<float>& a, hipStream_t stream);
+
+// Convenience for a single segment.
+template <typename T>
+inline GemmArgs<T> gemm_args(int M, int N, int K, T alpha, const T* A, long lda, const T* B,
+                             long ldb, int transb, T beta, T* C, long ldc) {
+    GemmArgs<T> g{};
+    g.seg[0] = {A, C, lda, ldc, M};
+    g.nseg = 1;
+    g.B = B;
+    g.ldb = ldb;
+    g.N = N;
+    g.K = K;
+    g.transb = transb;
+    g.tri = 0;
+    g.tri_off = 0;
+    g.alpha = alpha;
+    g.beta = beta;
+    return g;
+}
+
+}  // namespace moao

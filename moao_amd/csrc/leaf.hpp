@@ -1,0 +1,14 @@
+// Diagonal-leaf Cholesky + triangular inverse (see leaf.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace moao {
+
+constexpr int kLeaf = 128;  // leaf block size of the recursive factorization
+
+// Factor the nb x nb block at A (lower, in place) and write inv(L) to inv (kLeaf x kLeaf).
+int leaf_potrf_inv_launch(double* A, long lda, int nb, double* inv, int* info, int j0,
+                          hipStream_t stream);
+
+}  // namespace moao

@@ -1,0 +1,54 @@
+"""Minimal device-buffer handle over the C-ABI's allocator (no torch involved)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .lib import check, lib
+
+
+class DeviceArray:
+    """A column-major device matrix (or flat buffer) of float64/float32."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().moao_hip_malloc(ctypes.byref(p), self.nbytes), "moao_hip_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.asfortranarray(a)
+        d = cls(a.shape, a.dtype)
+        check(lib().moao_hip_memcpy_h2d(ctypes.c_void_p(d.ptr), a.ctypes.data, d.nbytes),
+              "moao_hip_memcpy_h2d")
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        check(lib().moao_hip_memcpy_d2h(out.ctypes.data, ctypes.c_void_p(self.ptr), self.nbytes),
+              "moao_hip_memcpy_d2h")
+        return out
+
+    def copy_from(self, other: "DeviceArray"):
+        assert other.nbytes <= self.nbytes
+        check(lib().moao_hip_memcpy_d2d(ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr),
+                                        other.nbytes), "moao_hip_memcpy_d2d")
+
+    def free(self):
+        if self.ptr:
+            lib().moao_hip_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(lib().moao_hip_synchronize(), "moao_hip_synchronize")

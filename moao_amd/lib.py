@@ -1,0 +1,123 @@
+"""Loader for libmoao_hip.so (the C-ABI declared in include/moao_hip.h).
+
+The product path has no fallback: if the shared library is missing or fails to load, every
+entry point raises.  Build it with ``python -m moao_amd.build`` (or __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libmoao_hip.so")
+
+MOAO_OK = 0
+MOAO_ERR_ARG = -1
+MOAO_ERR_HIP = -2
+MOAO_ERR_NOMEM = -3
+MOAO_ERR_NOCONV = -4
+MOAO_ERR_IO = -5
+MOAO_ERR_NODEV = -6
+
+MOAO_FP64 = 0
+MOAO_MIXED_FP32 = 1
+
+_c_long_p = ctypes.POINTER(ctypes.c_long)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class MoaoError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed with status {code}: {msg}")
+        self.code = code
+
+
+class TomoView(ctypes.Structure):
+    """ctypes mirror of moao_tomo_view (include/moao_hip.h)."""
+    _fields_ = [("Nw", ctypes.c_int), ("Nlayer", ctypes.c_int), ("nlgs", ctypes.c_int),
+                ("Nx", ctypes.c_long), ("diam", ctypes.c_double), ("lgs_cst", ctypes.c_double),
+                ("Nsubap", _c_long_p), ("Nssp", _c_long_p),
+                ("sspSizeL", _c_double_p), ("u", _c_double_p), ("v", _c_double_p),
+                ("X", _c_double_p), ("Y", _c_double_p), ("L0", _c_double_p),
+                ("cn2", _c_double_p), ("noiseNGS", _c_double_p), ("noiseLGSxx", _c_double_p),
+                ("noiseLGSyy", _c_double_p), ("noiseLGSxy", _c_double_p)]
+
+
+_lib = None
+
+# name -> (restype, argtypes); the exported C-ABI
+_SIGS = {
+    "moao_last_error": (ctypes.c_char_p, []),
+    "moao_version": (ctypes.c_char_p, []),
+    "moao_tomo_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "moao_tomo_update_atm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "moao_tomo_set_target": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "moao_tomo_set_target_coord": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_int]),
+    "moao_tomo_get_view": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TomoView)]),
+    "moao_tomo_nmeas": (ctypes.c_long, [ctypes.c_void_p]),
+    "moao_tomo_nmeasts": (ctypes.c_long, [ctypes.c_void_p]),
+    "moao_tomo_dims": (ctypes.c_int, [ctypes.c_void_p, _c_long_p]),
+    "moao_tomo_field": (ctypes.c_long, [ctypes.c_void_p, ctypes.c_char_p, _c_double_p,
+                                        ctypes.c_long]),
+    "moao_tomo_destroy": (None, [ctypes.c_void_p]),
+    "moao_hip_init": (ctypes.c_int, [ctypes.c_int]),
+    "moao_hip_finalize": (ctypes.c_int, []),
+    "moao_hip_malloc": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]),
+    "moao_hip_free": (ctypes.c_int, [ctypes.c_void_p]),
+    "moao_hip_memcpy_h2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "moao_hip_memcpy_d2h": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "moao_hip_memcpy_d2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "moao_hip_synchronize": (ctypes.c_int, []),
+    "moao_hip_mem_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t),
+                                         ctypes.POINTER(ctypes.c_size_t)]),
+    "moao_hip_matcov_tile": (ctypes.c_int, [_c_double_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(TomoView), ctypes.c_int]),
+    "moao_hip_matcov_tile_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.POINTER(TomoView), ctypes.c_int,
+                                                ctypes.c_void_p]),
+    "moao_hip_reconstructor": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, _c_double_p,
+                                              ctypes.c_long, _c_double_p, ctypes.c_long,
+                                              ctypes.c_long]),
+    "moao_hip_reconstructor_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_void_p,
+                                                  ctypes.c_long, ctypes.c_void_p, ctypes.c_long,
+                                                  ctypes.c_long, ctypes.c_void_p]),
+    "moao_hip_dgemm_dev": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_void_p, ctypes.c_long,
+                                          ctypes.c_void_p, ctypes.c_long, ctypes.c_double,
+                                          ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p]),
+    "moao_hip_sgemm_dev": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_float, ctypes.c_void_p, ctypes.c_long,
+                                          ctypes.c_void_p, ctypes.c_long, ctypes.c_float,
+                                          ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p]),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Load libmoao_hip.so once; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m moao_amd.build` (no CPU fallback)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc: int, where: str) -> int:
+    if rc < 0:
+        msg = lib().moao_last_error()
+        raise MoaoError(rc, where, msg.decode() if msg else "")
+    return rc

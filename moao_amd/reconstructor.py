@@ -1,0 +1,42 @@
+"""Reconstructor through the C-ABI: MOAO_LAPACK::reconstructor
+(lapack/compute/reconstructor.cpp:32-58)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .lib import check, lib
+
+
+def reconstructor(nmeas: int, nmeasts: int, Cmm: np.ndarray, ldCmm: int, Ctm: np.ndarray,
+                  ldCtm: int, ngal: int = 1) -> int:
+    """Host-buffer drop-in, in place: Cmm -> L (lower), Ctm blocks -> R.  Returns the dpotrf
+    info (0 on success).  Raises MoaoError on an argument/runtime error."""
+    for a in (Cmm, Ctm):
+        if a.dtype != np.float64 or not (a.flags.f_contiguous or a.ndim == 1):
+            raise TypeError("Cmm and Ctm must be column-major float64")
+    dp = ctypes.POINTER(ctypes.c_double)
+    return check(lib().moao_hip_reconstructor(nmeas, nmeasts, Cmm.ctypes.data_as(dp), ldCmm,
+                                              Ctm.ctypes.data_as(dp), ldCtm, ngal),
+                 "moao_hip_reconstructor")
+
+
+def reconstructor_dev(nmeas: int, nmeasts: int, dCmm: int, ldCmm: int, dCtm: int, ldCtm: int,
+                      ngal: int = 1, stream: int = 0) -> int:
+    return check(lib().moao_hip_reconstructor_dev(nmeas, nmeasts, ctypes.c_void_p(dCmm), ldCmm,
+                                                  ctypes.c_void_p(dCtm), ldCtm, ngal,
+                                                  ctypes.c_void_p(stream)),
+                 "moao_hip_reconstructor_dev")
+
+
+def dgemm_dev(transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, stream=0):
+    check(lib().moao_hip_dgemm_dev(transb, M, N, K, alpha, ctypes.c_void_p(dA), lda,
+                                   ctypes.c_void_p(dB), ldb, beta, ctypes.c_void_p(dC), ldc,
+                                   ctypes.c_void_p(stream)), "moao_hip_dgemm_dev")
+
+
+def sgemm_dev(transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, stream=0):
+    check(lib().moao_hip_sgemm_dev(transb, M, N, K, alpha, ctypes.c_void_p(dA), lda,
+                                   ctypes.c_void_p(dB), ldb, beta, ctypes.c_void_p(dC), ldc,
+                                   ctypes.c_void_p(stream)), "moao_hip_sgemm_dev")
