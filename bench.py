@@ -1,0 +1,189 @@
+"""Benchmark of the MOAO reconstructor hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload elt74] [--modes fp64,mixed]
+
+One step = one R-solve of the workload: restore the resident inputs (device-to-device copies of
+the Cmm and Ctm generated once by matcov) and solve R = Ctm * Cmm^{-1} in place.  The inputs
+are in HBM before the timed region.  Prints ONE JSON line (rank 0).
+
+value           R-solves/s of the headline path (mixed precision when built and accurate,
+                else FP64), timed over exactly K steps after W warm-up steps
+latency         p50 / p99 of the per-step wall time (each step synchronised)
+roofline        FP64 MFMA GEMM engine: algorithmic GEMM flops / summed GEMM kernel time
+                (HIP events on the launch stream) vs the 78.6 TFLOP/s FP64 peak
+cpu_baseline    the oracle's LAPACK path (scipy dpotrf + 2 dtrsm, OpenBLAS) timed on a bounded
+                leading sub-problem of the same matrices, scaled by flops to solves/s
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "R-solves/s + p50/p99 latency, mixed vs FP64 at <=1e-6 rel err, n=40k"
+PEAK_FP64 = 78.6e12
+PEAK_FP32 = 157.3e12
+
+
+def solve_flops(n, m):
+    """LAWN-41 POTRF + 2 right-side TRSM (chameleon/flops.h:72,80,364)."""
+    potrf = n * ((n / 6.0 + 0.5) * n + 1.0 / 3.0) + n * ((n / 6.0) * n - 1.0 / 6.0)
+    trsm = 2.0 * 0.5 * m * n * (n + 1.0)
+    return potrf + 2.0 * trsm
+
+
+def cpu_baseline(dCmm0, dCtm0, n, m, target_s=12.0):
+    """Oracle LAPACK path on the leading (ns x ns) block of Cmm and (ms x ns) of Ctm."""
+    from moao_amd.lib import lib
+    import ctypes
+    from oracle import lapack_path
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([p.get("num_threads", 1) for p in threadpool_info()
+                       if p.get("internal_api") in ("openblas", "mkl", "blis")] or [1])
+    except Exception:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # size the sample for ~target_s seconds at ~30 GFLOP/s per thread
+    rate_guess = 30e9 * threads
+    ns = int(min(n, (3.0 * target_s * rate_guess / (1 + 6 * m / n)) ** (1 / 3)))
+    ns = max(512, ns - ns % 64)
+    ms = max(1, int(round(m * ns / n)))
+    cols = np.empty((n, ns), order="F")
+    lib().moao_hip_memcpy_d2h(cols.ctypes.data, ctypes.c_void_p(dCmm0.ptr), cols.nbytes)
+    C = np.asfortranarray(cols[:ns, :ns])
+    del cols
+    bcols = np.empty((m, ns), order="F")
+    lib().moao_hip_memcpy_d2h(bcols.ctypes.data, ctypes.c_void_p(dCtm0.ptr), bcols.nbytes)
+    B = np.asfortranarray(bcols[:ms, :])
+    t0 = time.perf_counter()
+    info = lapack_path.reconstructor(C, B)
+    dt = time.perf_counter() - t0
+    rate = solve_flops(ns, ms) / dt
+    return {"value": rate / solve_flops(n, m), "unit": "R-solves/s", "cores": threads,
+            "kind": "port",
+            "sample": f"scipy/OpenBLAS dpotrf+2 dtrsm on the leading {ns}x{ns} Cmm block and "
+                      f"{ms}x{ns} Ctm block of the same matrices ({dt:.1f} s, "
+                      f"{rate / 1e9:.0f} GFLOP/s, info={info}); scaled by LAWN-41 flops to "
+                      f"n={n}, m={m}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="elt74")
+    ap.add_argument("--modes", default="fp64")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design note, not built)"}))
+        sys.exit(2)
+
+    from moao_amd import systems
+    from moao_amd.device import DeviceArray, Profile, synchronize
+    from moao_amd.lib import check, lib
+    from moao_amd.matcov import CMM, CTM, matcov_tile_dev
+    from moao_amd.reconstructor import reconstructor_dev
+    from moao_amd.tomo import Tomo
+
+    check(lib().moao_hip_init(0), "moao_hip_init")
+    tmp = tempfile.mkdtemp(prefix="moao_bench_")
+    sys_file, atms = systems.make_system(tmp, args.workload)
+    tomo = Tomo(sys_file, atms[0])
+    n, m = tomo.nmeas(), tomo.nmeasts()
+    print(f"[bench] workload {args.workload}: n={n} m={m}", file=sys.stderr, flush=True)
+
+    dCmm0 = DeviceArray((n, n))
+    dCtm0 = DeviceArray((m, n))
+    t0 = time.perf_counter()
+    with Profile() as prof_mc:
+        matcov_tile_dev(dCmm0.ptr, n, n, 0, 0, n, tomo, CMM)
+        tomo.set_target(0)
+        matcov_tile_dev(dCtm0.ptr, m, n, 0, 0, m, tomo, CTM)
+        synchronize()
+    t_matcov = time.perf_counter() - t0
+    mc = prof_mc.result["matcov"]
+    print(f"[bench] matcov {t_matcov:.2f} s ({mc['ms']:.1f} ms kernel)", file=sys.stderr,
+          flush=True)
+
+    dCmm, dCtm = DeviceArray((n, n)), DeviceArray((m, n))
+    results = {}
+    flops = solve_flops(n, m)
+    for mode in args.modes.split(","):
+        def step():
+            dCmm.copy_from(dCmm0)
+            dCtm.copy_from(dCtm0)
+            info = reconstructor_dev(n, m, dCmm.ptr, n, dCtm.ptr, m, 1)
+            if info != 0:
+                raise RuntimeError(f"reconstructor info={info}")
+            synchronize()
+        for _ in range(args.warmup):
+            step()
+        synchronize()
+        lat = []
+        with Profile() as prof:
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                s0 = time.perf_counter()
+                step()
+                lat.append(time.perf_counter() - s0)
+            synchronize()
+            total = time.perf_counter() - t0
+        results[mode] = {"total": total, "lat": lat, "prof": prof.result}
+        print(f"[bench] {mode}: {total / args.steps * 1e3:.1f} ms/solve", file=sys.stderr,
+              flush=True)
+
+    head = "mixed" if "mixed" in results else "fp64"
+    r = results[head]
+    ms_step = r["total"] / args.steps * 1e3
+    g = results["fp64"]["prof"]["gemm_f64"] if "fp64" in results else r["prof"]["gemm_f64"]
+    achieved = g["work"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
+    lat = sorted(r["lat"])
+    out = {
+        "metric": METRIC,
+        "value": args.steps / r["total"],
+        "unit": "R-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: seeded D=39 m system, n={n} meas, m={m} truth "
+                               f"meas, {tomo.dims()['nLayer']} layers, one R-solve per step",
+                   "n": n, "m": m, "path": head},
+        "latency_ms": {"p50": 1e3 * statistics.median(lat),
+                       "p99": 1e3 * lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]},
+        "solve_tflops": flops / (ms_step * 1e-3) / 1e12,
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                     "kernel": "gemm_kernel<double> (all launches of the timed solves)",
+                     "launches": g["count"], "avg_launch_ms": g["ms"] / max(1, g["count"])},
+        "stages_ms_per_solve": {k: v["ms"] / args.steps for k, v in r["prof"].items()
+                                if v["count"]},
+        "matcov_s": t_matcov,
+    }
+    for mode, rr in results.items():
+        out[f"{mode}_solves_per_s"] = args.steps / rr["total"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(dCmm0, dCtm0, n, m)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -135,6 +135,15 @@ int moao_hip_dgemm_dev(int transb, int M, int N, int K, double alpha, const doub
 int moao_hip_sgemm_dev(int transb, int M, int N, int K, float alpha, const float* A, long lda,
                        const float* B, long ldb, float beta, float* C, long ldc, void* stream);
 
+/* ---------------------------------------------------------------- timing ------------------
+ * Per-kernel-class timing with HIP events recorded on each launch's stream.  No reference
+ * counterpart (the reference times with gettimeofday around whole calls).  Classes:
+ * 0 FP64 GEMM, 1 FP32 GEMM, 2 leaf factor, 3 matcov, 4 precision conversion, 5 FFT, 6 other.
+ * work[] is the algorithmic work of the class: flops for GEMM/leaf, elements written for
+ * matcov, bytes moved for conversion/FFT passes.                                           */
+int moao_hip_profile_begin(void);
+int moao_hip_profile_end(double* ms, double* work, long* count);  /* arrays of 7 */
+
 #ifdef __cplusplus
 }
 #endif

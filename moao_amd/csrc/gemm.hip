@@ -9,8 +9,11 @@
 // and r = l/16 + 4t for f64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "gemm.hpp"
+#include "profile.hpp"
 
 namespace moao {
 namespace {
@@ -172,6 +175,19 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
         set_error("gemm: N too large for one launch");
         return MOAO_ERR_ARG;
     }
+    double work = 0;
+    if (profiling()) {
+        for (int i = 0; i < g.nseg; ++i) {
+            double area = (double)g.seg[i].M * g.N;
+            if (i == 0 && g.tri) {  // elements with row + tri_off >= col
+                area = 0;
+                for (int c = 0; c < g.N; ++c)
+                    area += std::max(0, g.seg[i].M - std::max(0, c - g.tri_off));
+            }
+            work += 2.0 * area * g.K;
+        }
+    }
+    ProfScope ps(sizeof(T) == 8 ? kProfGemmF64 : kProfGemmF32, work, stream);
     hipLaunchKernelGGL(gemm_kernel<T>, grid, dim3(NT), 0, stream, g);
     MOAO_HIP_TRY(hipGetLastError());
     return MOAO_OK;

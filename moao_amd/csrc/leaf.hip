@@ -11,6 +11,7 @@
 
 #include "common.hpp"
 #include "leaf.hpp"
+#include "profile.hpp"
 
 namespace moao {
 namespace {
@@ -86,6 +87,7 @@ int leaf_potrf_inv_launch(double* A, long lda, int nb, double* inv, int* info, i
         set_error("leaf_potrf: bad block size");
         return MOAO_ERR_ARG;
     }
+    ProfScope ps(kProfLeaf, (double)nb * nb * nb / 3.0, stream);
     hipLaunchKernelGGL(leaf_potrf_inv, dim3(1), dim3(NTL), 0, stream, A, lda, nb, inv, info, j0);
     MOAO_HIP_TRY(hipGetLastError());
     return MOAO_OK;

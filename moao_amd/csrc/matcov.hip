@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "profile.hpp"
 
 namespace moao {
 namespace {
@@ -362,10 +363,20 @@ int matcov_tile_launch(double* d_data, int nrows, int ncols, int xoffset, int yo
         set_error("matcov_tile: too many columns for one launch");
         return MOAO_ERR_ARG;
     }
-    if (type_mat == 4)
-        hipLaunchKernelGGL(truth_kernel, grid, dim3(kWaves * 64), 0, stream, a);
-    else
-        hipLaunchKernelGGL(meas_kernel, grid, dim3(kWaves * 64), 0, stream, a);
+    {
+        // work = elements written (bytes = 8 x elements)
+        double elems = (double)nrows * ncols;
+        if (type_mat == 1) {
+            elems = 0;
+            for (int c = 0; c < ncols; ++c)
+                elems += std::max(0, std::min(nrows, xoffset + nrows - (yoffset + c)));
+        }
+        ProfScope ps(kProfMatcov, elems, stream);
+        if (type_mat == 4)
+            hipLaunchKernelGGL(truth_kernel, grid, dim3(kWaves * 64), 0, stream, a);
+        else
+            hipLaunchKernelGGL(meas_kernel, grid, dim3(kWaves * 64), 0, stream, a);
+    }
     MOAO_HIP_TRY(hipGetLastError());
     // The staging buffer must not be rewritten by another call before this kernel ran.
     MOAO_HIP_TRY(hipStreamSynchronize(stream));

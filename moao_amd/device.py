@@ -52,3 +52,22 @@ class DeviceArray:
 
 def synchronize():
     check(lib().moao_hip_synchronize(), "moao_hip_synchronize")
+
+
+class Profile:
+    """Context manager collecting per-kernel-class event timings from the library."""
+
+    def __enter__(self):
+        check(lib().moao_hip_profile_begin(), "moao_hip_profile_begin")
+        return self
+
+    def __exit__(self, *exc):
+        from .lib import PROF_CLASSES
+        n = len(PROF_CLASSES)
+        ms = (ctypes.c_double * n)()
+        work = (ctypes.c_double * n)()
+        cnt = (ctypes.c_long * n)()
+        check(lib().moao_hip_profile_end(ms, work, cnt), "moao_hip_profile_end")
+        self.result = {c: {"ms": ms[i], "work": work[i], "count": cnt[i]}
+                       for i, c in enumerate(PROF_CLASSES)}
+        return False

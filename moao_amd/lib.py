@@ -93,7 +93,11 @@ _SIGS = {
                                           ctypes.c_float, ctypes.c_void_p, ctypes.c_long,
                                           ctypes.c_void_p, ctypes.c_long, ctypes.c_float,
                                           ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p]),
+    "moao_hip_profile_begin": (ctypes.c_int, []),
+    "moao_hip_profile_end": (ctypes.c_int, [_c_double_p, _c_double_p, _c_long_p]),
 }
+
+PROF_CLASSES = ["gemm_f64", "gemm_f32", "leaf", "matcov", "convert", "fft", "other"]
 
 
 def lib() -> ctypes.CDLL:
