@@ -126,9 +126,37 @@ int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, do
 int moao_hip_reconstructor_dev(long nmeas, long nmeasts, double* dCmm, long ldCmm, double* dCtm,
                                long ldCtm, long ngal, void* stream);
 
+/* Mixed precision (no reference counterpart; same result contract as the FP64 entry to a
+ * stated tolerance).  Cmm is read only (lower triangle) and preserved; the Ctm block (one
+ * target) is overwritten by R.  If `lowrank` describes the system Cmm was generated from, the
+ * exact lgs_cst low-rank term of the LGS blocks is removed before the FP32 factorization and
+ * restored by the Woodbury identity in FP64.  Refinement stops when
+ * ||Ctm - R Cmm||_F <= tol ||Ctm||_F (returned in *resid) or after max_iter corrections
+ * (then MOAO_ERR_NOCONV with R holding the last iterate).  mode: MOAO_MIXED_FP32.           */
+int moao_hip_reconstructor_mixed_dev(long nmeas, long nmeasts, const double* dCmm, long ldCmm,
+                                     double* dCtm, long ldCtm, const moao_tomo_view* lowrank,
+                                     int mode, double tol, int max_iter, double* resid,
+                                     int* iters, void* stream);
+
+/* ---------------------------------------------------------------- Cee / Cvv ---------------
+ * MOAO_LAPACK::Cee_Cvv (lapack/compute/cee_cvv.hpp:21, cee_cvv.cpp:50-124), FP64:
+ *   Cee = Ctt - Ctm R^T - R Ctm^T + R Cmm R^T  (m x m),  Tmp = Dx Cee,  Cvv = Tmp Dx^T.
+ * Cmm: regenerated covariance (lower triangle read).  Cpp (= Ctt) and Cpm (= Ctm) are
+ * clobbered as in the reference: Cpp's lower part holds Ctt - Ctm R^T - R Ctm^T and Cpm holds
+ * R Cmm on return.  Every ld argument is honoured.                                          */
+int moao_hip_cee_cvv(long nmeas, long nmeasts, long nact, double* Cmm, long ldCmm, double* Cpp,
+                     long ldCpp, double* Cpm, long ldCpm, double* R, long ldR, double* Dx,
+                     long ldDx, double* Cee, long ldCee, double* Cvv, long ldCvv, double* Tmp,
+                     long ldTmp);
+int moao_hip_cee_cvv_dev(long nmeas, long nmeasts, long nact, const double* dCmm, long ldCmm,
+                         double* dCpp, long ldCpp, double* dCpm, long ldCpm, const double* dR,
+                         long ldR, const double* dDx, long ldDx, double* dCee, long ldCee,
+                         double* dCvv, long ldCvv, double* dTmp, long ldTmp, void* stream);
+
 /* ---------------------------------------------------------------- GEMM utility ------------
  * C = alpha*A*op(B) + beta*C on the MFMA engine the stages use (column-major);
- * op(B) = B^T when transb != 0 (B is N x K), B otherwise (B is K x N).  No reference
+ * op(B) = B^T when transb == 1 (B is N x K), B when transb == 0 (B is K x N), and for
+ * transb == 2 the symmetric K x K matrix whose lower triangle B holds.  No reference
  * counterpart: exposed for tests and for callers composing their own chains.                */
 int moao_hip_dgemm_dev(int transb, int M, int N, int K, double alpha, const double* A, long lda,
                        const double* B, long ldb, double beta, double* C, long ldc, void* stream);

@@ -1,0 +1,142 @@
+// Cee / Cvv error covariances (MOAO_LAPACK::Cee_Cvv, lapack/compute/cee_cvv.cpp:50-124) on gfx950.
+//
+// The reference chain and what runs here, all FP64 on the MFMA GEMM engine:
+//   1 dsyr2k(L,N, -1, Cpm, R, 1, Cpp)       two lower-trapezoid GEMM launches (A B^T, B A^T)
+//   2 dsymm(R,L, 1, Cmm, R, 0, Cpm)         GEMM with B = Cmm read from its lower triangle
+//   3-5 dlaset(U) + dlacpy(L) + in-place geadd('T') + dscal(0.5) on the diagonal
+//                                           one kernel: Cee = lower(Cpp) mirrored (the net
+//                                           effect of those four calls)
+//   6 dgemm(N,T, 1, Cpm, R, 1, Cee)         GEMM
+//   7 dsymm(R,L, 1, Cee, Dx, 0, Tmp)        GEMM with B = Cee read from its lower triangle
+//   8 dgemm(N,T, 1, Tmp, Dx, 0, Cvv)        GEMM
+// so Cee = Ctt - Ctm R^T - R Ctm^T + R Cmm R^T and Cvv = Dx Cee Dx^T, with the reference's side
+// effects on Cpp (lower part = step 1 result) and Cpm (= R Cmm).  Unlike the reference, every
+// leading-dimension argument is honoured (the reference passes nmeasts/nmeas/nact regardless).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "profile.hpp"
+
+namespace moao {
+namespace {
+
+// Cee[i, j] = Cpp[max(i,j), min(i,j)]  (laset + lacpy + geadd('T') + scal on the diagonal)
+__global__ void mirror_lower(const double* __restrict__ src, long lds, double* __restrict__ dst,
+                             long ldd, int m) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= m || j >= m) return;
+    dst[i + (long)j * ldd] = i >= j ? src[i + (long)j * lds] : src[j + (long)i * lds];
+}
+
+}  // namespace
+
+int cee_cvv_dev(long n, long m, long nact, const double* Cmm, long ldCmm, double* Cpp,
+                long ldCpp, double* Cpm, long ldCpm, const double* R, long ldR,
+                const double* Dx, long ldDx, double* Cee, long ldCee, double* Cvv, long ldCvv,
+                double* Tmp, long ldTmp, hipStream_t s) {
+    if (n < 1 || m < 1 || nact < 0 || ldCmm < n || ldCpp < m || ldCpm < m || ldR < m ||
+        (nact > 0 && ldDx < nact) || ldCee < m || (nact > 0 && (ldCvv < nact || ldTmp < nact)) ||
+        n >= (1L << 31) || m >= (1L << 31) || nact >= (1L << 31)) {
+        set_error("cee_cvv: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    const int N = (int)n, M = (int)m, A = (int)nact;
+    // 1: Cpp(lower) -= Cpm R^T + R Cpm^T
+    auto g = gemm_args<double>(M, M, N, -1.0, Cpm, ldCpm, R, ldR, 1, 1.0, Cpp, ldCpp);
+    g.tri = 1;
+    if (int rc = gemm_f64(g, s)) return rc;
+    g = gemm_args<double>(M, M, N, -1.0, R, ldR, Cpm, ldCpm, 1, 1.0, Cpp, ldCpp);
+    g.tri = 1;
+    if (int rc = gemm_f64(g, s)) return rc;
+    // 2: Cpm = R sym(Cmm)
+    g = gemm_args<double>(M, N, N, 1.0, R, ldR, Cmm, ldCmm, 2, 0.0, Cpm, ldCpm);
+    if (int rc = gemm_f64(g, s)) return rc;
+    // 3-5: Cee = lower(Cpp) mirrored
+    {
+        ProfScope ps(kProfOther, 16.0 * m * m, s);
+        dim3 grid((M + 63) / 64, (M + 3) / 4);
+        hipLaunchKernelGGL(mirror_lower, grid, dim3(256), 0, s, Cpp, ldCpp, Cee, ldCee, M);
+        MOAO_HIP_TRY(hipGetLastError());
+    }
+    // 6: Cee += Cpm R^T
+    g = gemm_args<double>(M, M, N, 1.0, Cpm, ldCpm, R, ldR, 1, 1.0, Cee, ldCee);
+    if (int rc = gemm_f64(g, s)) return rc;
+    if (A == 0) return MOAO_OK;
+    // 7: Tmp = Dx sym(lower(Cee))
+    g = gemm_args<double>(A, M, M, 1.0, Dx, ldDx, Cee, ldCee, 2, 0.0, Tmp, ldTmp);
+    if (int rc = gemm_f64(g, s)) return rc;
+    // 8: Cvv = Tmp Dx^T
+    g = gemm_args<double>(A, A, M, 1.0, Tmp, ldTmp, Dx, ldDx, 1, 0.0, Cvv, ldCvv);
+    return gemm_f64(g, s);
+}
+
+}  // namespace moao
+
+extern "C" {
+
+int moao_hip_cee_cvv_dev(long nmeas, long nmeasts, long nact, const double* dCmm, long ldCmm,
+                         double* dCpp, long ldCpp, double* dCpm, long ldCpm, const double* dR,
+                         long ldR, const double* dDx, long ldDx, double* dCee, long ldCee,
+                         double* dCvv, long ldCvv, double* dTmp, long ldTmp, void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    return moao::cee_cvv_dev(nmeas, nmeasts, nact, dCmm, ldCmm, dCpp, ldCpp, dCpm, ldCpm, dR,
+                             ldR, dDx, ldDx, dCee, ldCee, dCvv, ldCvv, dTmp, ldTmp,
+                             moao::pick_stream(stream));
+}
+
+// Host-buffer drop-in: stages every operand through device memory and copies back Cpp, Cpm,
+// Cee, Cvv and Tmp (all of which the reference writes).
+int moao_hip_cee_cvv(long nmeas, long nmeasts, long nact, double* Cmm, long ldCmm, double* Cpp,
+                     long ldCpp, double* Cpm, long ldCpm, double* R, long ldR, double* Dx,
+                     long ldDx, double* Cee, long ldCee, double* Cvv, long ldCvv, double* Tmp,
+                     long ldTmp) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (!Cmm || !Cpp || !Cpm || !R || !Cee || (nact > 0 && (!Dx || !Cvv || !Tmp)) ||
+        nmeas < 1 || nmeasts < 1 || nact < 0) {
+        moao::set_error("cee_cvv: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    struct Op { double* h; long ld, rows, cols; bool in, out; size_t off; };
+    Op ops[] = {{Cmm, ldCmm, nmeas, nmeas, true, false, 0},
+                {Cpp, ldCpp, nmeasts, nmeasts, true, true, 0},
+                {Cpm, ldCpm, nmeasts, nmeas, true, true, 0},
+                {R, ldR, nmeasts, nmeas, true, false, 0},
+                {Dx, ldDx, nact, nmeasts, nact > 0, false, 0},
+                {Cee, ldCee, nmeasts, nmeasts, false, true, 0},
+                {Cvv, ldCvv, nact, nact, false, nact > 0, 0},
+                {Tmp, ldTmp, nact, nmeasts, false, nact > 0, 0}};
+    size_t total = 0;
+    for (Op& o : ops) {
+        o.off = total;
+        const size_t span = o.rows > 0 && o.cols > 0 ? (size_t)(o.cols - 1) * o.ld + o.rows : 0;
+        total += (span * sizeof(double) + 255) & ~size_t(255);
+    }
+    moao::Context& c = moao::ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    if (int rc = c.host_stage.ensure(total + 256)) return rc;
+    char* base = c.host_stage.as<char>();
+    hipStream_t s = c.stream;
+    auto span = [](const Op& o) {
+        return o.rows > 0 && o.cols > 0 ? ((size_t)(o.cols - 1) * o.ld + o.rows) * sizeof(double)
+                                         : 0;
+    };
+    // Outputs that leave gaps between columns (ld > rows) start from the caller's contents.
+    for (Op& o : ops)
+        if ((o.in || (o.out && o.ld != o.rows)) && span(o))
+            MOAO_HIP_TRY(hipMemcpyAsync(base + o.off, o.h, span(o), hipMemcpyHostToDevice, s));
+    auto D = [&](int i) { return (double*)(base + ops[i].off); };
+    int rc = moao::cee_cvv_dev(nmeas, nmeasts, nact, D(0), ldCmm, D(1), ldCpp, D(2), ldCpm, D(3),
+                               ldR, D(4), ldDx, D(5), ldCee, D(6), ldCvv, D(7), ldTmp, s);
+    if (rc) return rc;
+    for (Op& o : ops)
+        if (o.out && span(o))
+            MOAO_HIP_TRY(hipMemcpyAsync(o.h, base + o.off, span(o), hipMemcpyDeviceToHost, s));
+    MOAO_HIP_TRY(hipStreamSynchronize(s));
+    return MOAO_OK;
+}
+
+}  // extern "C"

@@ -45,7 +45,10 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int t) { return 4 * (lane >> 4) + t; }
 };
 
-template <typename T>
+// TB: 0 = B is K x N, 1 = B is N x K (op = transpose), 2 = B symmetric K x K, lower stored.
+// Loads are branch-free: addresses are clamped into the matrix and out-of-range values zeroed
+// by a select, so the compiler issues every load of a slice back to back.
+template <typename T, int TB>
 __global__ void __launch_bounds__(NT) gemm_kernel(GemmArgs<T> g) {
     using acc_t = typename Mfma<T>::acc_t;
     __shared__ T As[2][BK][BM + PAD];
@@ -63,6 +66,7 @@ __global__ void __launch_bounds__(NT) gemm_kernel(GemmArgs<T> g) {
     const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int K = g.K, N = g.N, M = s.M;
+    const int Mc = M - 1, Nc = N - 1, Kc = K > 0 ? K - 1 : 0;
 
     acc_t acc[4][4];
 #pragma unroll
@@ -71,48 +75,82 @@ __global__ void __launch_bounds__(NT) gemm_kernel(GemmArgs<T> g) {
         for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     // Global -> register staging.  A (and B^T): lane runs down 128 contiguous rows of a column.
+    // Raw loads go to registers with a validity mask; the mask is applied when the values are
+    // written to LDS (after the slice's MFMAs), so no wait sits between the loads.
     T ra[8], rb[8];
-    const int lrow = tid & 127, lk = tid >> 7;          // A / B^T pattern: 2 columns per pass
-    const int bk = tid & 15, bn = tid >> 4;             // B (K x N) pattern: 16 k per column
+    unsigned ma = 0, mb = 0;
+    const int lrow = tid & 127, lk = tid >> 7;  // A / B^T pattern: 2 columns per pass
+    const int bk = tid & 15, bn = tid >> 4;     // B (K x N) pattern: 16 k per column
+    const unsigned ulda = (unsigned)s.lda, uldb = (unsigned)g.ldb;
+    const int arow = m0 + lrow;
+    const T* __restrict__ Ap = s.A + min(arow, Mc);
+    const T* __restrict__ Bp = g.B;
+    const bool arow_ok = arow < M;
+    auto off = [](int i, unsigned ld) { return (unsigned long)(unsigned)i * ld; };
 
+    int bpat = TB == 1 ? 1 : 0;  // layout of the slice held in rb
     auto load = [&](int k0) {
-        const int row = m0 + lrow;
+        ma = mb = 0;
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int k = k0 + lk + 2 * r;
-            ra[r] = (row < M && k < K) ? s.A[row + (long)k * s.lda] : T(0);
+            ra[r] = Ap[off(min(k, Kc), ulda)];
+            ma |= (unsigned)(arow_ok && k < K) << r;
         }
-        if (g.transb) {
+        int pat = TB == 1 ? 1 : 0;
+        if (TB == 2) {
+            if (k0 + BK - 1 < n0) pat = 1;            // slice wholly above the diagonal
+            else if (k0 >= n0 + BN - 1) pat = 0;      // wholly below
+            else pat = 2;                             // crossing: per-element triangle
+        }
+        bpat = pat == 1 ? 1 : 0;
+        if (pat == 1) {
             const int col = n0 + lrow;
+            const T* __restrict__ bp = Bp + min(col, Nc);
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
                 const int k = k0 + lk + 2 * r;
-                rb[r] = (col < N && k < K) ? g.B[col + (long)k * g.ldb] : T(0);
+                rb[r] = bp[off(min(k, Kc), uldb)];
+                mb |= (unsigned)(col < N && k < K) << r;
             }
-        } else {
+        } else if (pat == 0) {
             const int k = k0 + bk;
+            const T* __restrict__ bp = Bp + min(k, Kc);
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
                 const int col = n0 + bn + 16 * r;
-                rb[r] = (col < N && k < K) ? g.B[k + (long)col * g.ldb] : T(0);
+                rb[r] = bp[off(min(col, Nc), uldb)];
+                mb |= (unsigned)(col < N && k < K) << r;
+            }
+        } else {
+            const int k = k0 + bk, kc = min(k, Kc);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int col = n0 + bn + 16 * r, cc = min(col, Nc);
+                rb[r] = kc >= cc ? Bp[kc + off(cc, uldb)] : Bp[cc + off(kc, uldb)];
+                mb |= (unsigned)(col < N && k < K) << r;
             }
         }
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) As[buf][lk + 2 * r][lrow] = ra[r];
-        if (g.transb) {
+        for (int r = 0; r < 8; ++r) As[buf][lk + 2 * r][lrow] = (ma >> r & 1) ? ra[r] : T(0);
+        if (bpat) {
 #pragma unroll
-            for (int r = 0; r < 8; ++r) Bs[buf][lk + 2 * r][lrow] = rb[r];
+            for (int r = 0; r < 8; ++r)
+                Bs[buf][lk + 2 * r][lrow] = (mb >> r & 1) ? rb[r] : T(0);
         } else {
 #pragma unroll
-            for (int r = 0; r < 8; ++r) Bs[buf][bk][bn + 16 * r] = rb[r];
+            for (int r = 0; r < 8; ++r)
+                Bs[buf][bk][bn + 16 * r] = (mb >> r & 1) ? rb[r] : T(0);
         }
     };
 
     const int nk = (K + BK - 1) / BK;
-    load(0);
-    store(0);
+    if (nk > 0) {
+        load(0);
+        store(0);
+    }
     __syncthreads();
     int cur = 0;
     const int ar = wm * 64 + (lane & 15);
@@ -163,7 +201,7 @@ __global__ void __launch_bounds__(NT) gemm_kernel(GemmArgs<T> g) {
 template <typename T>
 int launch(const GemmArgs<T>& g, hipStream_t stream) {
     if (g.nseg == 0) return MOAO_OK;
-    if (g.nseg < 0 || g.nseg > kMaxSeg || g.N < 0 || g.K < 0) {
+    if (g.nseg < 0 || g.nseg > kMaxSeg || g.N < 0 || g.K < 0 || g.transb < 0 || g.transb > 2) {
         set_error("gemm: invalid arguments");
         return MOAO_ERR_ARG;
     }
@@ -188,7 +226,12 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
         }
     }
     ProfScope ps(sizeof(T) == 8 ? kProfGemmF64 : kProfGemmF32, work, stream);
-    hipLaunchKernelGGL(gemm_kernel<T>, grid, dim3(NT), 0, stream, g);
+    if (g.transb == 0)
+        hipLaunchKernelGGL((gemm_kernel<T, 0>), grid, dim3(NT), 0, stream, g);
+    else if (g.transb == 1)
+        hipLaunchKernelGGL((gemm_kernel<T, 1>), grid, dim3(NT), 0, stream, g);
+    else
+        hipLaunchKernelGGL((gemm_kernel<T, 2>), grid, dim3(NT), 0, stream, g);
     MOAO_HIP_TRY(hipGetLastError());
     return MOAO_OK;
 }
@@ -208,7 +251,8 @@ int moao_hip_dgemm_dev(int transb, int M, int N, int K, double alpha, const doub
                        const double* B, long ldb, double beta, double* C, long ldc, void* stream) {
     if (int rc = moao::ensure_init()) return rc;
     if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldc < std::max(1, M) ||
-        ldb < std::max(1, transb ? N : K)) {
+        ldb < std::max(1, transb == 1 ? N : K) || transb < 0 || transb > 2 ||
+        (transb == 2 && K != N)) {
         moao::set_error("dgemm: invalid arguments");
         return MOAO_ERR_ARG;
     }
@@ -220,7 +264,8 @@ int moao_hip_sgemm_dev(int transb, int M, int N, int K, float alpha, const float
                        const float* B, long ldb, float beta, float* C, long ldc, void* stream) {
     if (int rc = moao::ensure_init()) return rc;
     if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldc < std::max(1, M) ||
-        ldb < std::max(1, transb ? N : K)) {
+        ldb < std::max(1, transb == 1 ? N : K) || transb < 0 || transb > 2 ||
+        (transb == 2 && K != N)) {
         moao::set_error("sgemm: invalid arguments");
         return MOAO_ERR_ARG;
     }

@@ -2,8 +2,10 @@
 //
 //   C_g[M_g x N] = alpha * A_g[M_g x K] * op(B) + beta * C_g       for each segment g
 //
-// Column-major, explicit leading dimensions.  op(B) = B^T with B stored N x K (transb = 1) or
-// op(B) = B with B stored K x N (transb = 0).  Up to kMaxSeg row segments share one B and one
+// Column-major, explicit leading dimensions.  op(B) = B^T with B stored N x K (transb = 1),
+// op(B) = B with B stored K x N (transb = 0), or op(B) = B for a symmetric K x K matrix of which
+// only the lower triangle is read (transb = 2, the B operand of dsymm(Right, Lower)).
+// Up to kMaxSeg row segments share one B and one
 // launch (grid.z = segment): the reconstructor uses this to update the Cmm rows and the Ctm
 // rows of every target with a single launch.  Segment 0 may be restricted to its lower
 // trapezoid (row + tri_off >= col): tiles wholly above that line are skipped and elements
@@ -44,6 +46,9 @@ struct GemmArgs {
 // Launches on `stream`; returns a MOAO_* status (argument checks only, launch is async).
 int gemm_f64(const GemmArgs<double>& a, hipStream_t stream);
 int gemm_f32(const GemmArgs<float>& a, hipStream_t stream);
+
+inline int gemm(const GemmArgs<double>& a, hipStream_t s) { return gemm_f64(a, s); }
+inline int gemm(const GemmArgs<float>& a, hipStream_t s) { return gemm_f32(a, s); }
 
 // Convenience for a single segment.
 template <typename T>

@@ -1,0 +1,358 @@
+// Mixed-precision reconstructor: R = Ctm * Cmm^{-1} to a stated tolerance with the O(n^3)
+// factorization and the O(m n^2) solves in FP32 and the residuals in FP64.
+//
+// Cmm carries an exact low-rank term: the reference adds lgs_cst to every entry of each LGS
+// xx and yy block (common/kernels/matcov_kernels.cpp:320-321), i.e. Cmm = C0 + c U U^T with U
+// the n x p indicator matrix of those 2*nlgs index ranges.  That term makes Cmm
+// ill-conditioned (its eigenvalues c*Nsubap sit 4-6 decades above the noise floor; FP32
+// Cholesky of Cmm loses definiteness by n ~ 1e4) while C0 stays well conditioned, so:
+//   1. C0 (FP32) = Cmm - c U U^T, converted in one pass;
+//   2. FP32 tall factorization of C0 with [Ctm; U^T] riding along, then the second solve:
+//      Y = Ctm C0^{-1}, W = U^T C0^{-1}   (FP32 MFMA updates, FP64 leaves);
+//   3. Woodbury in FP64: S = I/c + W U (p x p, host), R = Y - (Y U) S^{-1} W;
+//   4. refinement: E = Ctm - R Cmm (FP64 GEMM on the full Cmm, lower triangle read);
+//      stop when ||E||_F <= tol ||Ctm||_F, else D = E C0^{-1} (FP32 solves) corrected by
+//      Woodbury, R += D.
+// Without a low-rank description (p = 0) the same loop runs on Cmm itself.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "leaf.hpp"
+#include "profile.hpp"
+#include "recon.hpp"
+
+namespace moao {
+namespace {
+
+// C0[r,c] = Cmm[r,c] - c*[r,c in the same low-rank range], lower triangle, to FP32.
+__global__ void cmm_to_c0_f32(const double* __restrict__ A, long lda, float* __restrict__ B,
+                              long ldb, int n, const int* __restrict__ rid, double cst) {
+    const int r = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int c = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (r >= n || c >= n || r < c) return;
+    double v = A[r + (long)c * lda];
+    const int a = rid[r];
+    if (a >= 0 && a == rid[c]) v -= cst;
+    B[r + (long)c * ldb] = (float)v;
+}
+
+__global__ void cvt_f64_f32(const double* __restrict__ a, long lda, float* __restrict__ b,
+                            long ldb, int rows, int cols) {
+    const int r = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int c = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (r < rows && c < cols) b[r + (long)c * ldb] = (float)a[r + (long)c * lda];
+}
+
+__global__ void cvt_f32_f64(const float* __restrict__ a, long lda, double* __restrict__ b,
+                            long ldb, int rows, int cols) {
+    const int r = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int c = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (r < rows && c < cols) b[r + (long)c * ldb] = (double)a[r + (long)c * lda];
+}
+
+// b += a (FP64)
+__global__ void add_f64(const double* __restrict__ a, long lda, double* __restrict__ b, long ldb,
+                        int rows, int cols) {
+    const int r = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int c = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (r < rows && c < cols) b[r + (long)c * ldb] += a[r + (long)c * lda];
+}
+
+// Per-workgroup partial sums of squares of a rows x cols matrix (one column strip per group).
+__global__ void sumsq_partial(const double* __restrict__ a, long lda, int rows, int cols,
+                              double* __restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    const long total = (long)rows * cols;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const long c = e / rows, r = e - c * rows;
+        const double v = a[r + c * lda];
+        acc += v * v;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+constexpr int kNormBlocks = 1024;
+
+dim3 grid2(int rows, int cols) { return dim3((rows + 63) / 64, (cols + 3) / 4); }
+
+struct Ranges {
+    std::vector<long> start, len;
+    double cst = 0.0;
+    int p() const { return (int)start.size(); }
+};
+
+Ranges lowrank_ranges(const moao_tomo_view* t, long n) {
+    Ranges R;
+    if (!t || t->lgs_cst == 0.0) return R;
+    R.cst = t->lgs_cst;
+    long off = 0;
+    for (int w = 0; w < t->Nw - 1; ++w) {
+        const long ns = t->Nsubap[w];
+        if (w < t->nlgs) {
+            R.start.push_back(2 * off);
+            R.len.push_back(ns);
+            R.start.push_back(2 * off + ns);
+            R.len.push_back(ns);
+        }
+        off += ns;
+    }
+    if (2 * off != n) R.start.clear(), R.len.clear();  // view does not describe this Cmm
+    return R;
+}
+
+// Solve p x p SPD system S X = B (column-major, in place) by Cholesky; false if not SPD.
+bool small_spd_inverse(std::vector<double>& S, int p, std::vector<double>& inv) {
+    std::vector<double> L(S);
+    for (int j = 0; j < p; ++j) {
+        double d = L[j + j * p];
+        for (int k = 0; k < j; ++k) d -= L[j + k * p] * L[j + k * p];
+        if (!(d > 0)) return false;
+        d = std::sqrt(d);
+        L[j + j * p] = d;
+        for (int i = j + 1; i < p; ++i) {
+            double v = L[i + j * p];
+            for (int k = 0; k < j; ++k) v -= L[i + k * p] * L[j + k * p];
+            L[i + j * p] = v / d;
+        }
+    }
+    inv.assign((size_t)p * p, 0.0);
+    for (int c = 0; c < p; ++c) {
+        std::vector<double> x(p, 0.0);
+        x[c] = 1.0;
+        for (int i = 0; i < p; ++i) {  // L y = e_c
+            double v = x[i];
+            for (int k = 0; k < i; ++k) v -= L[i + k * p] * x[k];
+            x[i] = v / L[i + i * p];
+        }
+        for (int i = p - 1; i >= 0; --i) {  // L^T z = y
+            double v = x[i];
+            for (int k = i + 1; k < p; ++k) v -= L[k + i * p] * x[k];
+            x[i] = v / L[i + i * p];
+        }
+        for (int i = 0; i < p; ++i) inv[i + c * p] = x[i];
+    }
+    return true;
+}
+
+struct MixedWs {
+    float* C0;     // n x n
+    float* X32;    // (m + p) x n: rows [0,m) right-hand sides, rows [m, m+p) U^T
+    float* invL;   // leaves
+    double* E;     // m x n residual / correction
+    double* Ctm0;  // m x n copy of the right-hand side for the residuals
+    double* U;     // n x p indicator
+    double* W;     // p x n
+    double* Z;     // m x p
+    double* Z2;    // m x p
+    double* Sinv;  // p x p
+    double* WU;    // p x p
+    double* part;  // norm partial sums
+    int* rid;      // n range ids
+    int* info;
+};
+
+}  // namespace
+
+int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, double* Ctm, long ldCtm,
+                            const moao_tomo_view* lowrank, double tol, int max_iter,
+                            double* resid_out, int* iters_out, hipStream_t s) {
+    if (n < 1 || m < 1 || ldCmm < n || ldCtm < m || !Cmm || !Ctm || max_iter < 0 ||
+        n >= (1L << 31) || m >= (1L << 31)) {
+        set_error("reconstructor_mixed: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    const Ranges rg = lowrank_ranges(lowrank, n);
+    const int p = rg.p();
+    const long xr = m + p;  // rows of the FP32 right-hand-side block
+    // workspace layout
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t off = 0;
+    std::vector<size_t> offs;
+    auto take = [&](size_t bytes) { offs.push_back(off); off += al(bytes); };
+    take(sizeof(float) * n * n);
+    take(sizeof(float) * xr * n);
+    take(sizeof(float) * recon_invl_doubles(n));
+    take(sizeof(double) * m * n);
+    take(sizeof(double) * m * n);
+    take(sizeof(double) * n * std::max(p, 1));
+    take(sizeof(double) * std::max(p, 1) * n);
+    take(sizeof(double) * m * std::max(p, 1));
+    take(sizeof(double) * m * std::max(p, 1));
+    take(sizeof(double) * std::max(p * p, 1));
+    take(sizeof(double) * std::max(p * p, 1));
+    take(sizeof(double) * kNormBlocks);
+    take(sizeof(int) * n);
+    take(sizeof(int) * 4);
+    Context& c = ctx();
+    if (int rc = c.mixed_ws.ensure(off)) return rc;
+    char* b = c.mixed_ws.as<char>();
+    MixedWs w{(float*)(b + offs[0]), (float*)(b + offs[1]), (float*)(b + offs[2]),
+              (double*)(b + offs[3]), (double*)(b + offs[4]), (double*)(b + offs[5]),
+              (double*)(b + offs[6]), (double*)(b + offs[7]), (double*)(b + offs[8]),
+              (double*)(b + offs[9]), (double*)(b + offs[10]), (double*)(b + offs[11]),
+              (int*)(b + offs[12]), (int*)(b + offs[13])};
+    const int N = (int)n, M = (int)m;
+
+    auto norm = [&](const double* a, long lda, int rows, int cols, double* out) -> int {
+        hipLaunchKernelGGL(sumsq_partial, dim3(kNormBlocks), dim3(256), 0, s, a, lda, rows, cols,
+                           w.part);
+        MOAO_HIP_TRY(hipGetLastError());
+        std::vector<double> h(kNormBlocks);
+        MOAO_HIP_TRY(hipMemcpyAsync(h.data(), w.part, sizeof(double) * kNormBlocks,
+                                    hipMemcpyDeviceToHost, s));
+        MOAO_HIP_TRY(hipStreamSynchronize(s));
+        double acc = 0;
+        for (double v : h) acc += v;
+        *out = std::sqrt(acc);
+        return MOAO_OK;
+    };
+
+    // --- range ids, U, U^T
+    {
+        std::vector<int> rid(n, -1);
+        std::vector<double> U((size_t)n * std::max(p, 1), 0.0);
+        std::vector<float> Ut((size_t)std::max(p, 1) * n, 0.0f);
+        for (int q = 0; q < p; ++q)
+            for (long k = rg.start[q]; k < rg.start[q] + rg.len[q]; ++k) {
+                rid[k] = q;
+                U[k + (size_t)q * n] = 1.0;
+            }
+        MOAO_HIP_TRY(hipMemcpyAsync(w.rid, rid.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        if (p) {
+            MOAO_HIP_TRY(hipMemcpyAsync(w.U, U.data(), sizeof(double) * n * p,
+                                        hipMemcpyHostToDevice, s));
+            // U^T rows live below the m right-hand-side rows of X32 (leading dimension xr)
+            std::vector<float> Xu((size_t)p * n);
+            for (int q = 0; q < p; ++q)
+                for (long k = 0; k < n; ++k) Xu[q + (size_t)k * p] = (float)U[k + (size_t)q * n];
+            MOAO_HIP_TRY(hipMemcpy2DAsync(w.X32 + m, xr * sizeof(float), Xu.data(),
+                                          p * sizeof(float), p * sizeof(float), n,
+                                          hipMemcpyHostToDevice, s));
+        }
+        MOAO_HIP_TRY(hipStreamSynchronize(s));  // host vectors go out of scope
+    }
+
+    // --- 1. C0 in FP32; right-hand sides Ctm in FP32
+    {
+        ProfScope ps(kProfConvert, 12.0 * n * (n + 1) / 2, s);
+        hipLaunchKernelGGL(cmm_to_c0_f32, grid2(N, N), dim3(256), 0, s, Cmm, ldCmm, w.C0, n, N,
+                           w.rid, rg.cst);
+        hipLaunchKernelGGL(cvt_f64_f32, grid2(M, N), dim3(256), 0, s, Ctm, ldCtm, w.X32, xr, M, N);
+        MOAO_HIP_TRY(hipGetLastError());
+    }
+    double ctm_norm = 0;
+    if (int rc = norm(Ctm, ldCtm, M, N, &ctm_norm)) return rc;
+    // keep the original Ctm for the residuals
+    MOAO_HIP_TRY(hipMemcpy2DAsync(w.Ctm0, m * sizeof(double), Ctm, ldCtm * sizeof(double),
+                                  m * sizeof(double), n, hipMemcpyDeviceToDevice, s));
+
+    // --- 2. FP32 factorization with [Ctm; U^T] riding along, second solve
+    std::vector<RowGroup<float>> all{{w.X32, xr, (int)xr}};
+    if (int rc = potrf_tall<float>(n, w.C0, n, all, w.invL, w.info, s)) return rc;
+    int hinfo = 0;
+    MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
+    MOAO_HIP_TRY(hipStreamSynchronize(s));
+    if (hinfo) {
+        set_error("reconstructor_mixed: FP32 factorization of C0 failed");
+        return hinfo;
+    }
+    if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, all, s)) return rc;
+
+    // --- 3. Woodbury terms
+    std::vector<double> Sinv;
+    if (p) {
+        hipLaunchKernelGGL(cvt_f32_f64, grid2(p, N), dim3(256), 0, s, w.X32 + m, xr, w.W, p, p, N);
+        MOAO_HIP_TRY(hipGetLastError());
+        auto g = gemm_args<double>(p, p, N, 1.0, w.W, p, w.U, n, 0, 0.0, w.WU, p);
+        if (int rc = gemm_f64(g, s)) return rc;
+        std::vector<double> S((size_t)p * p);
+        MOAO_HIP_TRY(hipMemcpyAsync(S.data(), w.WU, sizeof(double) * p * p, hipMemcpyDeviceToHost,
+                                    s));
+        MOAO_HIP_TRY(hipStreamSynchronize(s));
+        for (int q = 0; q < p; ++q) S[q + q * p] += 1.0 / rg.cst;
+        for (int i = 0; i < p; ++i)  // symmetrise the FP64 product
+            for (int j = 0; j < i; ++j) S[i + j * p] = S[j + i * p] = 0.5 * (S[i + j * p] + S[j + i * p]);
+        if (!small_spd_inverse(S, p, Sinv)) {
+            set_error("reconstructor_mixed: Woodbury capacitance matrix not SPD");
+            return MOAO_ERR_NOCONV;
+        }
+        MOAO_HIP_TRY(hipMemcpyAsync(w.Sinv, Sinv.data(), sizeof(double) * p * p,
+                                    hipMemcpyHostToDevice, s));
+    }
+    // Y (FP64) -> X; X -= (X U) S^{-1} W
+    auto woodbury = [&](double* X, long ldx) -> int {
+        if (!p) return MOAO_OK;
+        auto g = gemm_args<double>(M, p, N, 1.0, X, ldx, w.U, n, 0, 0.0, w.Z, m);
+        if (int rc = gemm_f64(g, s)) return rc;
+        g = gemm_args<double>(M, p, p, 1.0, w.Z, m, w.Sinv, p, 0, 0.0, w.Z2, m);
+        if (int rc = gemm_f64(g, s)) return rc;
+        g = gemm_args<double>(M, N, p, -1.0, w.Z2, m, w.W, p, 0, 1.0, X, ldx);
+        return gemm_f64(g, s);
+    };
+    hipLaunchKernelGGL(cvt_f32_f64, grid2(M, N), dim3(256), 0, s, w.X32, xr, Ctm, ldCtm, M, N);
+    MOAO_HIP_TRY(hipGetLastError());
+    if (int rc = woodbury(Ctm, ldCtm)) return rc;  // Ctm now holds R_0
+
+    // --- 4. refinement
+    double rel = 0;
+    int it = 0;
+    int status = MOAO_OK;
+    for (;; ++it) {
+        MOAO_HIP_TRY(hipMemcpyAsync(w.E, w.Ctm0, sizeof(double) * m * n, hipMemcpyDeviceToDevice,
+                                    s));
+        auto g = gemm_args<double>(M, N, N, -1.0, Ctm, ldCtm, Cmm, ldCmm, 2, 1.0, w.E, m);
+        if (int rc = gemm_f64(g, s)) { status = rc; break; }
+        double en = 0;
+        if (int rc = norm(w.E, m, M, N, &en)) { status = rc; break; }
+        rel = ctm_norm > 0 ? en / ctm_norm : en;
+        if (rel <= tol) break;
+        if (it >= max_iter) { status = MOAO_ERR_NOCONV; break; }
+        // D = E C0^{-1} (FP32 solves), Woodbury-corrected in FP64, R += D
+        hipLaunchKernelGGL(cvt_f64_f32, grid2(M, N), dim3(256), 0, s, w.E, m, w.X32, xr, M, N);
+        std::vector<RowGroup<float>> rhs{{w.X32, xr, M}};
+        if (int rc = trsm_right_lower_trans<float>(n, w.C0, n, w.invL, rhs, s)) { status = rc; break; }
+        if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, rhs, s)) { status = rc; break; }
+        hipLaunchKernelGGL(cvt_f32_f64, grid2(M, N), dim3(256), 0, s, w.X32, xr, w.E, m, M, N);
+        if (int rc = woodbury(w.E, m)) { status = rc; break; }
+        hipLaunchKernelGGL(add_f64, grid2(M, N), dim3(256), 0, s, w.E, m, Ctm, ldCtm, M, N);
+        MOAO_HIP_TRY(hipGetLastError());
+    }
+    MOAO_HIP_TRY(hipStreamSynchronize(s));
+    if (resid_out) *resid_out = rel;
+    if (iters_out) *iters_out = it;
+    if (status == MOAO_ERR_NOCONV)
+        set_error("reconstructor_mixed: refinement stopped at relative residual " +
+                  std::to_string(rel));
+    return status;
+}
+
+}  // namespace moao
+
+extern "C" {
+
+int moao_hip_reconstructor_mixed_dev(long nmeas, long nmeasts, const double* dCmm, long ldCmm,
+                                     double* dCtm, long ldCtm, const moao_tomo_view* lowrank,
+                                     int mode, double tol, int max_iter, double* resid,
+                                     int* iters, void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (mode != MOAO_MIXED_FP32) {
+        moao::set_error("reconstructor_mixed: unsupported mode");
+        return MOAO_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> g(moao::ctx().mu);
+    return moao::reconstructor_mixed_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, lowrank, tol,
+                                         max_iter, resid, iters, moao::pick_stream(stream));
+}
+
+}  // extern "C"

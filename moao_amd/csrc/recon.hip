@@ -6,12 +6,14 @@
 //    [Cmm; Ctm_0; ...; Ctm_{ngal-1}] (only the top n x n block is SPD): every update GEMM also
 //    updates the Ctm rows, so X = Ctm * L^{-T} falls out of the factorization;
 //  * the second solve Y * L = X runs backwards by the same recursion;
-//  * leaves (kLeaf columns) are factored and inverted in LDS; their panel solves are in-place
-//    GEMMs with the inverse, so every flop except the leaves runs on MFMA.
+//  * leaves (kLeaf columns) are factored and inverted in LDS in FP64; their panel solves are
+//    in-place GEMMs with the inverse, so every flop except the leaves runs on MFMA.
 // The split point of every recursion is a multiple of kLeaf, so the top-level updates are the
-// largest GEMMs (K = n/2) and the deep levels stay cheap.
+// largest GEMMs (K = n/2) and the deep levels stay cheap.  The same recursion runs on FP32
+// storage (FP32 MFMA updates, FP64 leaves) for the mixed-precision path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -28,147 +30,224 @@ size_t recon_invl_doubles(long n) {
 
 namespace {
 
-struct Problem {
-    long n, m, ngal;
-    double* Cmm;
-    long ldA;
-    double* Ctm;
-    long ldB;
-    double* invL;
-    int* info;
-    hipStream_t s;
-    double* ctm(long g) const { return Ctm + g * n * m; }
-    double* cmm(long r, long c) const { return Cmm + r + c * ldA; }
-};
-
 long split(long nn) {
     long n1 = ((nn / 2 + kLeaf - 1) / kLeaf) * kLeaf;
     if (n1 >= nn) n1 = nn - kLeaf;
     return n1;
 }
 
-// Target groups of one launch: segment 0 is the Cmm rows (if any), then Ctm targets.
-template <typename F>
-int for_target_chunks(const Problem& p, int first_free, F&& fn) {
-    long g = 0;
-    do {
-        const int cap = kMaxSeg - first_free;
-        const long cnt = std::min<long>(cap, p.ngal - g);
-        if (int rc = fn(g, (int)cnt)) return rc;
-        g += cnt;
-        first_free = 0;
-    } while (g < p.ngal);
+template <typename T>
+struct Tall {
+    long n;
+    T* A;
+    long lda;
+    const std::vector<RowGroup<T>>* groups;
+    T* invL;
+    int* info;
+    hipStream_t s;
+    T* a(long r, long c) const { return A + r + c * lda; }
+    T* inv(long j0) const { return invL + (j0 / kLeaf) * (long)kLeaf * kLeaf; }
+};
+
+// Launch one GEMM over [optional lead segment] + every row group, kMaxSeg segments at a time.
+// make(q) fills the segment of group q (column offsets applied by the caller).
+template <typename T, typename Lead, typename Make>
+int over_groups(const std::vector<RowGroup<T>>& groups, GemmArgs<T> proto, bool has_lead,
+                Lead&& lead, Make&& make, hipStream_t s) {
+    size_t q = 0;
+    bool lead_pending = has_lead;
+    while (lead_pending || q < groups.size()) {
+        GemmArgs<T> a = proto;
+        int ns = 0;
+        a.tri = 0;
+        if (lead_pending) {
+            a.seg[ns++] = lead();
+            a.tri = proto.tri;
+            lead_pending = false;
+        }
+        while (ns < kMaxSeg && q < groups.size()) {
+            if (groups[q].rows > 0) a.seg[ns++] = make(groups[q]);
+            ++q;
+        }
+        a.nseg = ns;
+        if (ns)
+            if (int rc = gemm(a, s)) return rc;
+    }
     return MOAO_OK;
 }
 
 // Factor columns [j0, j0+nn) of the tall matrix (all earlier columns already applied).
-int potrf_rec(const Problem& p, long j0, long nn) {
+template <typename T>
+int potrf_rec(const Tall<T>& p, long j0, long nn) {
+    const auto& G = *p.groups;
     if (nn <= kLeaf) {
-        double* inv = p.invL + (j0 / kLeaf) * (long)kLeaf * kLeaf;
-        if (int rc = leaf_potrf_inv_launch(p.cmm(j0, j0), p.ldA, (int)nn, inv, p.info, (int)j0, p.s))
+        if (int rc = leaf_potrf_inv_launch<T>(p.a(j0, j0), p.lda, (int)nn, p.inv(j0), p.info,
+                                              (int)j0, p.s))
             return rc;
-        // Panel: rows below the leaf (Cmm) and every Ctm row, times inv(L)^T, in place.
+        // Panel: rows below the leaf and every group row, times inv(L)^T, in place.
         const long below = p.n - j0 - nn;
-        bool cmm_done = below <= 0;
-        return for_target_chunks(p, cmm_done ? 0 : 1, [&](long g0, int cnt) {
-            GemmArgs<double> a{};
-            int ns = 0;
-            if (!cmm_done) {
-                a.seg[ns++] = {p.cmm(j0 + nn, j0), p.cmm(j0 + nn, j0), p.ldA, p.ldA, (int)below};
-                cmm_done = true;
-            }
-            for (int q = 0; q < cnt; ++q) {
-                double* c = p.ctm(g0 + q) + j0 * p.ldB;
-                a.seg[ns++] = {c, c, p.ldB, p.ldB, (int)p.m};
-            }
-            a.nseg = ns;
-            a.B = inv;
-            a.ldb = kLeaf;
-            a.N = (int)nn;
-            a.K = (int)nn;
-            a.transb = 1;
-            a.alpha = 1.0;
-            a.beta = 0.0;
-            return gemm_f64(a, p.s);
-        });
+        GemmArgs<T> proto{};
+        proto.B = p.inv(j0);
+        proto.ldb = kLeaf;
+        proto.N = (int)nn;
+        proto.K = (int)nn;
+        proto.transb = 1;
+        proto.alpha = T(1);
+        proto.beta = T(0);
+        return over_groups<T>(
+            G, proto, below > 0,
+            [&] { return GemmSeg<T>{p.a(j0 + nn, j0), p.a(j0 + nn, j0), p.lda, p.lda, (int)below}; },
+            [&](const RowGroup<T>& g) {
+                T* c = g.p + j0 * g.ld;
+                return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
+            },
+            p.s);
     }
     const long n1 = split(nn);
     if (int rc = potrf_rec(p, j0, n1)) return rc;
-    // Columns [j0+n1, j0+nn) of all rows below j0+n1 -= rows * Cmm[j0+n1:j0+nn, j0:j0+n1]^T
+    // Columns [j0+n1, j0+nn) of every row below j0+n1 -= rows * A[j0+n1:j0+nn, j0:j0+n1]^T
     const long r0 = j0 + n1;
-    bool cmm_done = false;
-    int rc = for_target_chunks(p, 1, [&](long g0, int cnt) {
-        GemmArgs<double> a{};
-        int ns = 0;
-        if (!cmm_done) {
-            a.seg[ns++] = {p.cmm(r0, j0), p.cmm(r0, r0), p.ldA, p.ldA, (int)(p.n - r0)};
-            a.tri = 1;
-            a.tri_off = 0;
-            cmm_done = true;
-        }
-        for (int q = 0; q < cnt; ++q) {
-            double* c = p.ctm(g0 + q);
-            a.seg[ns++] = {c + j0 * p.ldB, c + r0 * p.ldB, p.ldB, p.ldB, (int)p.m};
-        }
-        a.nseg = ns;
-        a.B = p.cmm(r0, j0);
-        a.ldb = p.ldA;
-        a.N = (int)(nn - n1);
-        a.K = (int)n1;
-        a.transb = 1;
-        a.alpha = -1.0;
-        a.beta = 1.0;
-        return gemm_f64(a, p.s);
-    });
+    GemmArgs<T> proto{};
+    proto.B = p.a(r0, j0);
+    proto.ldb = p.lda;
+    proto.N = (int)(nn - n1);
+    proto.K = (int)n1;
+    proto.transb = 1;
+    proto.tri = 1;
+    proto.tri_off = 0;
+    proto.alpha = T(-1);
+    proto.beta = T(1);
+    int rc = over_groups<T>(
+        G, proto, true,
+        [&] { return GemmSeg<T>{p.a(r0, j0), p.a(r0, r0), p.lda, p.lda, (int)(p.n - r0)}; },
+        [&](const RowGroup<T>& g) {
+            return GemmSeg<T>{g.p + j0 * g.ld, g.p + r0 * g.ld, g.ld, g.ld, g.rows};
+        },
+        p.s);
     if (rc) return rc;
     return potrf_rec(p, r0, nn - n1);
 }
 
-// Solve Y * L = X for columns [j0, j0+nn) given contributions of Y[:, j0+nn:] already removed.
-int trsm2_rec(const Problem& p, long j0, long nn) {
+// Y * L = X for columns [j0, j0+nn), contributions of Y[:, j0+nn:] already removed.
+template <typename T>
+int trsmN_rec(const Tall<T>& p, long j0, long nn) {
+    const auto& G = *p.groups;
+    auto none = [] { return GemmSeg<T>{}; };
     if (nn <= kLeaf) {
-        const double* inv = p.invL + (j0 / kLeaf) * (long)kLeaf * kLeaf;
-        return for_target_chunks(p, 0, [&](long g0, int cnt) {
-            GemmArgs<double> a{};
-            for (int q = 0; q < cnt; ++q) {
-                double* c = p.ctm(g0 + q) + j0 * p.ldB;
-                a.seg[q] = {c, c, p.ldB, p.ldB, (int)p.m};
-            }
-            a.nseg = cnt;
-            a.B = inv;
-            a.ldb = kLeaf;
-            a.N = (int)nn;
-            a.K = (int)nn;
-            a.transb = 0;
-            a.alpha = 1.0;
-            a.beta = 0.0;
-            return gemm_f64(a, p.s);
-        });
+        GemmArgs<T> proto{};
+        proto.B = p.inv(j0);
+        proto.ldb = kLeaf;
+        proto.N = (int)nn;
+        proto.K = (int)nn;
+        proto.transb = 0;
+        proto.alpha = T(1);
+        proto.beta = T(0);
+        return over_groups<T>(G, proto, false, none,
+                              [&](const RowGroup<T>& g) {
+                                  T* c = g.p + j0 * g.ld;
+                                  return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
+                              },
+                              p.s);
     }
     const long n1 = split(nn);
-    if (int rc = trsm2_rec(p, j0 + n1, nn - n1)) return rc;
+    if (int rc = trsmN_rec(p, j0 + n1, nn - n1)) return rc;
     // X[:, j0:j0+n1] -= Y[:, j0+n1:j0+nn] * L[j0+n1:j0+nn, j0:j0+n1]
-    int rc = for_target_chunks(p, 0, [&](long g0, int cnt) {
-        GemmArgs<double> a{};
-        for (int q = 0; q < cnt; ++q) {
-            double* c = p.ctm(g0 + q);
-            a.seg[q] = {c + (j0 + n1) * p.ldB, c + j0 * p.ldB, p.ldB, p.ldB, (int)p.m};
-        }
-        a.nseg = cnt;
-        a.B = p.cmm(j0 + n1, j0);
-        a.ldb = p.ldA;
-        a.N = (int)n1;
-        a.K = (int)(nn - n1);
-        a.transb = 0;
-        a.alpha = -1.0;
-        a.beta = 1.0;
-        return gemm_f64(a, p.s);
-    });
+    GemmArgs<T> proto{};
+    proto.B = p.a(j0 + n1, j0);
+    proto.ldb = p.lda;
+    proto.N = (int)n1;
+    proto.K = (int)(nn - n1);
+    proto.transb = 0;
+    proto.alpha = T(-1);
+    proto.beta = T(1);
+    int rc = over_groups<T>(G, proto, false, none,
+                            [&](const RowGroup<T>& g) {
+                                return GemmSeg<T>{g.p + (j0 + n1) * g.ld, g.p + j0 * g.ld, g.ld,
+                                                  g.ld, g.rows};
+                            },
+                            p.s);
     if (rc) return rc;
-    return trsm2_rec(p, j0, n1);
+    return trsmN_rec(p, j0, n1);
+}
+
+// X * L^T = B for columns [j0, j0+nn), contributions of X[:, :j0] already removed.
+template <typename T>
+int trsmT_rec(const Tall<T>& p, long j0, long nn) {
+    const auto& G = *p.groups;
+    auto none = [] { return GemmSeg<T>{}; };
+    if (nn <= kLeaf) {
+        GemmArgs<T> proto{};
+        proto.B = p.inv(j0);
+        proto.ldb = kLeaf;
+        proto.N = (int)nn;
+        proto.K = (int)nn;
+        proto.transb = 1;
+        proto.alpha = T(1);
+        proto.beta = T(0);
+        return over_groups<T>(G, proto, false, none,
+                              [&](const RowGroup<T>& g) {
+                                  T* c = g.p + j0 * g.ld;
+                                  return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
+                              },
+                              p.s);
+    }
+    const long n1 = split(nn);
+    if (int rc = trsmT_rec(p, j0, n1)) return rc;
+    // B[:, j0+n1:j0+nn] -= X[:, j0:j0+n1] * L[j0+n1:j0+nn, j0:j0+n1]^T
+    GemmArgs<T> proto{};
+    proto.B = p.a(j0 + n1, j0);
+    proto.ldb = p.lda;
+    proto.N = (int)(nn - n1);
+    proto.K = (int)n1;
+    proto.transb = 1;
+    proto.alpha = T(-1);
+    proto.beta = T(1);
+    int rc = over_groups<T>(G, proto, false, none,
+                            [&](const RowGroup<T>& g) {
+                                return GemmSeg<T>{g.p + j0 * g.ld, g.p + (j0 + n1) * g.ld, g.ld,
+                                                  g.ld, g.rows};
+                            },
+                            p.s);
+    if (rc) return rc;
+    return trsmT_rec(p, j0 + n1, nn - n1);
 }
 
 }  // namespace
+
+template <typename T>
+int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
+               int* info, hipStream_t s) {
+    Tall<T> p{n, A, lda, &groups, invL, info, s};
+    MOAO_HIP_TRY(hipMemsetAsync(info, 0, sizeof(int), s));
+    return potrf_rec(p, 0, n);
+}
+
+template <typename T>
+int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
+                             const std::vector<RowGroup<T>>& groups, hipStream_t s) {
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, s};
+    return trsmN_rec(p, 0, n);
+}
+
+template <typename T>
+int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
+                           const std::vector<RowGroup<T>>& groups, hipStream_t s) {
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, s};
+    return trsmT_rec(p, 0, n);
+}
+
+template int potrf_tall<double>(long, double*, long, const std::vector<RowGroup<double>>&,
+                                double*, int*, hipStream_t);
+template int potrf_tall<float>(long, float*, long, const std::vector<RowGroup<float>>&, float*,
+                               int*, hipStream_t);
+template int trsm_right_lower_notrans<double>(long, const double*, long, const double*,
+                                              const std::vector<RowGroup<double>>&, hipStream_t);
+template int trsm_right_lower_notrans<float>(long, const float*, long, const float*,
+                                             const std::vector<RowGroup<float>>&, hipStream_t);
+template int trsm_right_lower_trans<double>(long, const double*, long, const double*,
+                                            const std::vector<RowGroup<double>>&, hipStream_t);
+template int trsm_right_lower_trans<float>(long, const float*, long, const float*,
+                                           const std::vector<RowGroup<float>>&, hipStream_t);
 
 int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm, long ldCtm,
                           long ngal, double* dinvL, int* dinfo, hipStream_t stream) {
@@ -177,16 +256,17 @@ int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm
         set_error("reconstructor: invalid arguments");
         return MOAO_ERR_ARG;
     }
-    Problem p{n, m, ngal, dCmm, ldCmm, dCtm, ldCtm, dinvL, dinfo, stream};
-    if (m == 0) p.ngal = 0;
-    MOAO_HIP_TRY(hipMemsetAsync(dinfo, 0, sizeof(int), stream));
-    if (int rc = potrf_rec(p, 0, n)) return rc;
+    std::vector<RowGroup<double>> groups;
+    if (m > 0)
+        for (long g = 0; g < ngal; ++g) groups.push_back({dCtm + g * n * m, ldCtm, (int)m});
+    if (int rc = potrf_tall<double>(n, dCmm, ldCmm, groups, dinvL, dinfo, stream)) return rc;
     int hinfo = 0;
     MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, stream));
     MOAO_HIP_TRY(hipStreamSynchronize(stream));
     if (hinfo != 0) return hinfo;
     if (m > 0)
-        if (int rc = trsm2_rec(p, 0, n)) return rc;
+        if (int rc = trsm_right_lower_notrans<double>(n, dCmm, ldCmm, dinvL, groups, stream))
+            return rc;
     return MOAO_OK;
 }
 

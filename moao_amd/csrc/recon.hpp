@@ -3,16 +3,44 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
+#include "leaf.hpp"
+
 namespace moao {
+
+// A block of rows that rides along the factorization as extra rows of the tall matrix
+// [Cmm; group_0; group_1; ...]: after potrf_tall each group holds X = B * L^{-T}.
+template <typename T>
+struct RowGroup {
+    T* p;
+    long ld;
+    int rows;
+};
+
+// Recursive factorization of the n x n SPD block (lower, in place) with the row groups
+// solved by L^T on the way (first right-side solve fused).  invL receives the inverse of every
+// kLeaf leaf (ceil(n/kLeaf)*kLeaf*kLeaf elements); *info the dpotrf status (device int).
+template <typename T>
+int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
+               int* info, hipStream_t s);
+// Y * L = X for every group (second right-side solve), with L / invL from potrf_tall.
+template <typename T>
+int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
+                             const std::vector<RowGroup<T>>& groups, hipStream_t s);
+// X * L^T = B for every group (first right-side solve, standalone).
+template <typename T>
+int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
+                           const std::vector<RowGroup<T>>& groups, hipStream_t s);
 
 // FP64 path.  dCmm (n x n, lower) is overwritten by its Cholesky factor L; each of the ngal
 // target blocks Ctm_g = dCtm + g*n*m (m x n, leading dimension ldCtm) is overwritten by R_g.
-// On a non-positive pivot *info = its 1-based order and the Ctm blocks hold partial results.
-// dinvL: workspace of ceil(n/kLeaf)*kLeaf*kLeaf doubles; dinfo: one device int.
+// On a non-positive pivot the return value is its 1-based order and the Ctm blocks hold
+// partial results.  dinvL: workspace of recon_invl_elems(n) doubles; dinfo: one device int.
 int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm, long ldCtm,
                           long ngal, double* dinvL, int* dinfo, hipStream_t stream);
 
-// Size in doubles of the dinvL workspace.
+// Number of elements of the inverse-leaf workspace.
 size_t recon_invl_doubles(long n);
 
 }  // namespace moao

@@ -229,6 +229,7 @@ int moao_hip_finalize(void) {
     c.host_stage.release();
     c.host_stage2.release();
     c.recon_ws.release();
+    c.mixed_ws.release();
     return MOAO_OK;
 }
 

@@ -85,6 +85,14 @@ _SIGS = {
     "moao_hip_reconstructor_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_void_p,
                                                   ctypes.c_long, ctypes.c_void_p, ctypes.c_long,
                                                   ctypes.c_long, ctypes.c_void_p]),
+    "moao_hip_reconstructor_mixed_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
+                                                        ctypes.c_void_p, ctypes.c_long,
+                                                        ctypes.c_void_p, ctypes.c_long,
+                                                        ctypes.POINTER(TomoView), ctypes.c_int,
+                                                        ctypes.c_double, ctypes.c_int,
+                                                        _c_double_p,
+                                                        ctypes.POINTER(ctypes.c_int),
+                                                        ctypes.c_void_p]),
     "moao_hip_dgemm_dev": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, ctypes.c_void_p, ctypes.c_long,
                                           ctypes.c_void_p, ctypes.c_long, ctypes.c_double,
@@ -93,6 +101,10 @@ _SIGS = {
                                           ctypes.c_float, ctypes.c_void_p, ctypes.c_long,
                                           ctypes.c_void_p, ctypes.c_long, ctypes.c_float,
                                           ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p]),
+    "moao_hip_cee_cvv": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_long]
+                         + [_c_double_p, ctypes.c_long] * 8),
+    "moao_hip_cee_cvv_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_long]
+                             + [ctypes.c_void_p, ctypes.c_long] * 8 + [ctypes.c_void_p]),
     "moao_hip_profile_begin": (ctypes.c_int, []),
     "moao_hip_profile_end": (ctypes.c_int, [_c_double_p, _c_double_p, _c_long_p]),
 }

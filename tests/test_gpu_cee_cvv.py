@@ -1,0 +1,72 @@
+"""GPU parity of Cee/Cvv with the reference's lapack chain (oracle/lapack_path.cee_cvv:
+scipy dsyr2k, dsymm, laset/lacpy/geadd/scal, dgemm, dsymm, dgemm in the reference's order).
+
+Tolerances: Cee is a small difference of large terms (|Cee| ~ 1e-3 |Ctt| on these systems), so
+its relative error is the FP64 rounding of the O(|Ctt|) terms divided by |Cee|: 1e-10 relative
+Frobenius; Cvv = Dx Cee Dx^T inherits it (1e-10); the side effects (Cpm = R Cmm, lower Cpp) are
+O(1) quantities: 1e-12.
+"""
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, fixture_system
+from moao_amd import matcov as gm
+from moao_amd.cee_cvv import cee_cvv
+from moao_amd.systems import dx_matrix
+from moao_amd.tomo import Tomo
+from oracle import lapack_path as lp
+from oracle.check import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_cee_cvv_matches_lapack_chain(name):
+    print(f"[cee] fixture {name}", flush=True)
+    t = Tomo(*fixture_system(name))
+    C = gm.cmm(t)
+    t.set_target(0)
+    Ctm, Ctt = gm.ctm(t), gm.ctt(t)
+    n, m = C.shape[0], Ctt.shape[0]
+    R = Ctm.copy(order="F")
+    lp.reconstructor(C.copy(order="F"), R)
+    nact = 137
+    Dx = dx_matrix(nact, m, n)
+    # oracle
+    Cpp_o, Cpm_o = Ctt.copy(order="F"), Ctm.copy(order="F")
+    Cee_o, Cvv_o = lp.cee_cvv(C, Cpp_o, Cpm_o, R, Dx)
+    # GPU
+    Cpp, Cpm = Ctt.copy(order="F"), Ctm.copy(order="F")
+    Cee = np.zeros((m, m), order="F")
+    Cvv = np.zeros((nact, nact), order="F")
+    Tmp = np.zeros((nact, m), order="F")
+    cee_cvv(n, m, nact, C, Cpp, Cpm, R, Dx, Cee, Cvv, Tmp)
+    e_cee, e_cvv = rel_fro(Cee, Cee_o), rel_fro(Cvv, Cvv_o)
+    e_cpm = rel_fro(Cpm, Cpm_o)
+    low = np.tril_indices(m)
+    e_cpp = rel_fro(Cpp[low], Cpp_o[low])
+    print(f"    Cee {e_cee:.2e} Cvv {e_cvv:.2e} Cpm {e_cpm:.2e} Cpp {e_cpp:.2e}", flush=True)
+    assert e_cee <= 1e-10 and e_cvv <= 1e-10
+    assert e_cpm <= 1e-12 and e_cpp <= 1e-12
+    # upper part of Cpp keeps the caller's values (dsyr2k 'L')
+    up = np.triu_indices(m, 1)
+    assert np.array_equal(Cpp[up], Ctt[up])
+    assert np.linalg.norm(Tmp - Dx @ Cee_o) / np.linalg.norm(Tmp) <= 1e-10
+
+
+def test_symmetric_b_gemm_mode():
+    """transb=2 reads only the lower triangle of B (upper filled with garbage)."""
+    from moao_amd.device import DeviceArray
+    from moao_amd.reconstructor import dgemm_dev
+    rng = np.random.default_rng(4)
+    for M, K in [(50, 300), (257, 129), (7, 16)]:
+        print(f"[cee] symmetric-B gemm M={M} K={K}", flush=True)
+        S = rng.standard_normal((K, K))
+        S = S + S.T
+        B = np.tril(S) + np.triu(np.full((K, K), 1e300), 1)
+        A = rng.standard_normal((M, K))
+        dA, dB = DeviceArray.from_host(A), DeviceArray.from_host(np.asfortranarray(B))
+        dC = DeviceArray.from_host(np.zeros((M, K), order="F"))
+        dgemm_dev(2, M, K, K, 1.0, dA.ptr, M, dB.ptr, K, 0.0, dC.ptr, M)
+        ref = A @ S
+        assert np.max(np.abs(dC.to_host() - ref)) <= 1e-12 * np.max(np.abs(ref))

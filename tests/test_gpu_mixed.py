@@ -1,0 +1,73 @@
+"""Mixed-precision reconstructor (FP32 factorization of C0 = Cmm - lgs_cst U U^T, Woodbury in
+FP64, FP64 refinement against the full Cmm) against the FP64 lapack-path oracle.
+
+Bar: relative Frobenius error of R <= 1e-6 (the accuracy the mixed path is specified to), with
+the refinement stopping at ||Ctm - R Cmm|| / ||Ctm|| <= 1e-11; Cmm must be left untouched.
+"""
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, fixture_system
+from moao_amd import matcov as gm
+from moao_amd import systems
+from moao_amd.device import DeviceArray
+from moao_amd.reconstructor import reconstructor_mixed_dev
+from moao_amd.tomo import Tomo
+from oracle import lapack_path as lp
+from oracle.check import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(t, use_lowrank=True, tol=1e-11, max_iter=6):
+    C = gm.cmm(t)
+    t.set_target(0)
+    B = gm.ctm(t)
+    n, m = C.shape[0], B.shape[0]
+    Ro = B.copy(order="F")
+    assert lp.reconstructor(C.copy(order="F"), Ro) == 0
+    dC, dB = DeviceArray.from_host(C), DeviceArray.from_host(B)
+    rc, res, its = reconstructor_mixed_dev(n, m, dC.ptr, n, dB.ptr, m,
+                                           t if use_lowrank else None, tol, max_iter)
+    R = dB.to_host()
+    assert np.array_equal(dC.to_host(), C)  # Cmm preserved
+    return rel_fro(R, Ro), res, its
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_mixed_fixture_systems(name):
+    print(f"[mixed] fixture {name}", flush=True)
+    err, res, its = _run(Tomo(*fixture_system(name)))
+    print(f"    err={err:.2e} resid={res:.2e} steps={its}", flush=True)
+    assert res <= 1e-11 and err <= 1e-6
+
+
+def test_mixed_small_workload(tmp_path):
+    print("[mixed] seeded small workload", flush=True)
+    s, atms = systems.make_system(str(tmp_path), "small")
+    err, res, its = _run(Tomo(s, atms[0]))
+    print(f"    err={err:.2e} resid={res:.2e} steps={its}", flush=True)
+    assert res <= 1e-11 and err <= 1e-6
+
+
+def test_mixed_without_lowrank_split(tmp_path):
+    """No low-rank description: FP32 factorization of Cmm itself (small n, still SPD in FP32)."""
+    print("[mixed] no low-rank split", flush=True)
+    s, atms = systems.make_system(str(tmp_path), "small")
+    err, res, its = _run(Tomo(s, atms[0]), use_lowrank=False, max_iter=10)
+    print(f"    err={err:.2e} resid={res:.2e} steps={its}", flush=True)
+    assert res <= 1e-11 and err <= 1e-6
+
+
+def test_mixed_reports_nonconvergence(tmp_path):
+    from moao_amd.lib import MOAO_ERR_NOCONV
+    print("[mixed] max_iter=0 reports non-convergence", flush=True)
+    s, atms = systems.make_system(str(tmp_path), "small")
+    t = Tomo(s, atms[0])
+    C = gm.cmm(t)
+    t.set_target(0)
+    B = gm.ctm(t)
+    dC, dB = DeviceArray.from_host(C), DeviceArray.from_host(B)
+    rc, res, its = reconstructor_mixed_dev(C.shape[0], B.shape[0], dC.ptr, C.shape[0], dB.ptr,
+                                           B.shape[0], t, 1e-15, 0, allow_noconv=True)
+    assert rc == MOAO_ERR_NOCONV and its == 0 and res > 1e-15
