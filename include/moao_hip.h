@@ -153,6 +153,25 @@ int moao_hip_cee_cvv_dev(long nmeas, long nmeasts, long nact, const double* dCmm
                          long ldR, const double* dDx, long ldDx, double* dCee, long ldCee,
                          double* dCvv, long ldCvv, double* dTmp, long ldTmp, void* stream);
 
+/* ---------------------------------------------------------------- intersample PSF ---------
+ * intersample_prepare/init/process/free (common/tomo_struct/intersample.hpp:46-63,
+ * intersample.cpp:72-333).  The reference reads idx/abs2fi/otftel from FITS files; here the
+ * caller passes the arrays (idx: nidx 1-based map indices; abs2fi, otf_tel: N*N in the FITS /
+ * reference flat layout).  N <= 0 selects the reference's rule (moao_intersample_default_n).
+ * precision: 0 = FP32 FFTs (default product path), 1 = FP64 FFTs.
+ * process: Cvv is the flat nact x nact covariance (nidx doubles), the PSF (N*N doubles, the
+ * reference's `dphi` output layout) is written to psf; the _dev entry takes `batch` stacked
+ * Cvv / PSF arrays (one per direction) in device memory.                                   */
+typedef struct moao_isample moao_isample;
+long moao_intersample_default_n(long np);
+int moao_hip_intersample_create(long nidx, long np, long N, const long* idx, const double* abs2fi,
+                                const double* otf_tel, int precision, moao_isample** out);
+long moao_hip_intersample_n(const moao_isample* s);
+int moao_hip_intersample_process(moao_isample* s, const double* Cvv, double* psf);
+int moao_hip_intersample_process_dev(moao_isample* s, const double* dCvv, int batch,
+                                     double* dPsf, void* stream);
+void moao_hip_intersample_destroy(moao_isample* s);
+
 /* ---------------------------------------------------------------- GEMM utility ------------
  * C = alpha*A*op(B) + beta*C on the MFMA engine the stages use (column-major);
  * op(B) = B^T when transb == 1 (B is N x K), B when transb == 0 (B is K x N), and for

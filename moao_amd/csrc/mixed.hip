@@ -16,7 +16,11 @@
 // Without a low-rank description (p = 0) the same loop runs on Cmm itself.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -111,36 +115,38 @@ Ranges lowrank_ranges(const moao_tomo_view* t, long n) {
     return R;
 }
 
-// Solve p x p SPD system S X = B (column-major, in place) by Cholesky; false if not SPD.
-bool small_spd_inverse(std::vector<double>& S, int p, std::vector<double>& inv) {
-    std::vector<double> L(S);
-    for (int j = 0; j < p; ++j) {
-        double d = L[j + j * p];
-        for (int k = 0; k < j; ++k) d -= L[j + k * p] * L[j + k * p];
-        if (!(d > 0)) return false;
-        d = std::sqrt(d);
-        L[j + j * p] = d;
-        for (int i = j + 1; i < p; ++i) {
-            double v = L[i + j * p];
-            for (int k = 0; k < j; ++k) v -= L[i + k * p] * L[j + k * p];
-            L[i + j * p] = v / d;
-        }
-    }
+// Inverse of the p x p matrix S (column-major) by Gauss-Jordan with partial pivoting; false if
+// singular.  S = I/c + W U is used exactly as computed (not symmetrised): the Woodbury identity
+// reproduces the inverse of C0_low + c U U^T only when S is formed from the same W that is
+// applied, and W from low-precision solves makes W U symmetric only to ~1e-5.
+bool small_inverse(const std::vector<double>& S, int p, std::vector<double>& inv) {
+    std::vector<double> a(S);
     inv.assign((size_t)p * p, 0.0);
+    for (int i = 0; i < p; ++i) inv[i + i * p] = 1.0;
     for (int c = 0; c < p; ++c) {
-        std::vector<double> x(p, 0.0);
-        x[c] = 1.0;
-        for (int i = 0; i < p; ++i) {  // L y = e_c
-            double v = x[i];
-            for (int k = 0; k < i; ++k) v -= L[i + k * p] * x[k];
-            x[i] = v / L[i + i * p];
+        int piv = c;
+        for (int r = c + 1; r < p; ++r)
+            if (std::fabs(a[r + c * p]) > std::fabs(a[piv + c * p])) piv = r;
+        if (!(std::fabs(a[piv + c * p]) > 0)) return false;
+        if (piv != c)
+            for (int k = 0; k < p; ++k) {
+                std::swap(a[c + k * p], a[piv + k * p]);
+                std::swap(inv[c + k * p], inv[piv + k * p]);
+            }
+        const double d = a[c + c * p];
+        for (int k = 0; k < p; ++k) {
+            a[c + k * p] /= d;
+            inv[c + k * p] /= d;
         }
-        for (int i = p - 1; i >= 0; --i) {  // L^T z = y
-            double v = x[i];
-            for (int k = i + 1; k < p; ++k) v -= L[k + i * p] * x[k];
-            x[i] = v / L[i + i * p];
+        for (int r = 0; r < p; ++r) {
+            if (r == c) continue;
+            const double f = a[r + c * p];
+            if (f == 0.0) continue;
+            for (int k = 0; k < p; ++k) {
+                a[r + k * p] -= f * a[c + k * p];
+                inv[r + k * p] -= f * inv[c + k * p];
+            }
         }
-        for (int i = 0; i < p; ++i) inv[i + c * p] = x[i];
     }
     return true;
 }
@@ -172,6 +178,7 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
         set_error("reconstructor_mixed: invalid arguments");
         return MOAO_ERR_ARG;
     }
+    const bool debug = std::getenv("MOAO_DEBUG") != nullptr;
     const Ranges rg = lowrank_ranges(lowrank, n);
     const int p = rg.p();
     const long xr = m + p;  // rows of the FP32 right-hand-side block
@@ -281,11 +288,14 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
                                     s));
         MOAO_HIP_TRY(hipStreamSynchronize(s));
         for (int q = 0; q < p; ++q) S[q + q * p] += 1.0 / rg.cst;
-        for (int i = 0; i < p; ++i)  // symmetrise the FP64 product
-            for (int j = 0; j < i; ++j) S[i + j * p] = S[j + i * p] = 0.5 * (S[i + j * p] + S[j + i * p]);
-        if (!small_spd_inverse(S, p, Sinv)) {
-            set_error("reconstructor_mixed: Woodbury capacitance matrix not SPD");
+        if (!small_inverse(S, p, Sinv)) {
+            set_error("reconstructor_mixed: Woodbury capacitance matrix is singular");
             return MOAO_ERR_NOCONV;
+        }
+        if (debug) {
+            double tr = 0, trs = 0;
+            for (int q = 0; q < p; ++q) tr += S[q + q * p], trs += Sinv[q + q * p];
+            std::fprintf(stderr, "[mixed] p=%d trace(S)=%.6e trace(S^-1)=%.6e\n", p, tr, trs);
         }
         MOAO_HIP_TRY(hipMemcpyAsync(w.Sinv, Sinv.data(), sizeof(double) * p * p,
                                     hipMemcpyHostToDevice, s));
@@ -302,7 +312,21 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
     };
     hipLaunchKernelGGL(cvt_f32_f64, grid2(M, N), dim3(256), 0, s, w.X32, xr, Ctm, ldCtm, M, N);
     MOAO_HIP_TRY(hipGetLastError());
+    if (debug) {
+        double a = 0, b2 = 0, c2 = 0;
+        norm(Ctm, ldCtm, M, N, &a);
+        if (p) norm(w.W, p, p, N, &b2);
+        if (p) norm(w.U, n, N, p, &c2);
+        std::fprintf(stderr, "[mixed] |Ctm|=%.6e |Y|=%.6e |W|=%.6e |U|=%.6e\n", ctm_norm, a, b2, c2);
+    }
     if (int rc = woodbury(Ctm, ldCtm)) return rc;  // Ctm now holds R_0
+    if (debug && p) {
+        double a = 0, b2 = 0, c2 = 0;
+        norm(w.Z, m, M, p, &a);
+        norm(w.Z2, m, M, p, &b2);
+        norm(Ctm, ldCtm, M, N, &c2);
+        std::fprintf(stderr, "[mixed] |Z|=%.6e |Z2|=%.6e |R0|=%.6e\n", a, b2, c2);
+    }
 
     // --- 4. refinement
     double rel = 0;
@@ -316,6 +340,7 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
         double en = 0;
         if (int rc = norm(w.E, m, M, N, &en)) { status = rc; break; }
         rel = ctm_norm > 0 ? en / ctm_norm : en;
+        if (debug) std::fprintf(stderr, "[mixed] step %d relative residual %.3e\n", it, rel);
         if (rel <= tol) break;
         if (it >= max_iter) { status = MOAO_ERR_NOCONV; break; }
         // D = E C0^{-1} (FP32 solves), Woodbury-corrected in FP64, R += D

@@ -1,0 +1,79 @@
+"""GPU intersample PSF against the oracle's FP64 numpy restatement (FFTW r2c/c2r semantics).
+
+FP64 FFT path: PSF within 1e-11 of the peak (only summation order differs).
+FP32 FFT path (the product default): PSF within 2e-5 of the peak and |Strehl - Strehl_64| <= 1e-5
+(FP32 transforms of the phase-structure map; the cancellation norm - D is done in FP64).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from moao_amd import psf_inputs as pi
+from moao_amd.device import DeviceArray
+from moao_amd.lib import check, lib
+from oracle import intersample as oi
+
+pytestmark = pytest.mark.gpu
+
+dp = ctypes.POINTER(ctypes.c_double)
+lp = ctypes.POINTER(ctypes.c_long)
+
+
+def _inputs(nssp, N, diam, scale, seed=0):
+    dm = pi.make_dm(nssp, 0.2, 0.28)
+    idx = pi.make_idx(dm)
+    a2f = pi.make_abs2fi(N, diam, dm)
+    otf = pi.make_otf_tel(N, diam, 0.28)
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((dm.nact, dm.nact))
+    cvv = np.ascontiguousarray((G @ G.T / dm.nact * scale).flatten(order="F"))
+    return dm, idx, a2f, otf, cvv
+
+
+def _create(idx, np_, N, a2f, otf, prec):
+    h = ctypes.c_void_p()
+    check(lib().moao_hip_intersample_create(idx.size, np_, N, idx.ctypes.data_as(lp),
+                                            a2f.ctypes.data_as(dp), otf.ctypes.data_as(dp), prec,
+                                            ctypes.byref(h)), "intersample_create")
+    return h
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 2e-5)])
+@pytest.mark.parametrize("nssp,N", [(10, 512), (16, 256)])
+def test_psf_matches_oracle(prec, tol, nssp, N):
+    print(f"[psf] prec={prec} nssp={nssp} N={N}", flush=True)
+    dm, idx, a2f, otf, cvv = _inputs(nssp, N, 8.0, 3e-4)
+    np_ = dm.ndiam
+    ref = oi.process(cvv, idx, a2f, otf, N, np_)
+    h = _create(idx, np_, N, a2f, otf, prec)
+    psf = np.zeros(N * N)
+    check(lib().moao_hip_intersample_process(h, cvv.ctypes.data_as(dp), psf.ctypes.data_as(dp)),
+          "intersample_process")
+    lib().moao_hip_intersample_destroy(h)
+    psf = psf.reshape(N, N)
+    peak = np.max(np.abs(ref))
+    err = np.max(np.abs(psf - ref)) / peak
+    ds = abs(psf[N // 2, N // 2] - ref[N // 2, N // 2])
+    print(f"    strehl={ref[N // 2, N // 2]:.6f} err={err:.2e} dStrehl={ds:.2e}", flush=True)
+    assert err <= tol
+    assert ds <= (1e-12 if prec else 1e-5)
+
+
+def test_batched_directions_match_single():
+    print("[psf] batch of 3 directions", flush=True)
+    N, nssp = 256, 10
+    dm, idx, a2f, otf, cvv0 = _inputs(nssp, N, 8.0, 3e-4, seed=1)
+    np_ = dm.ndiam
+    cvvs = [cvv0, 0.5 * cvv0, np.zeros_like(cvv0)]
+    h = _create(idx, np_, N, a2f, otf, 0)
+    dC = DeviceArray.from_host(np.concatenate(cvvs))
+    dP = DeviceArray((3 * N * N,))
+    check(lib().moao_hip_intersample_process_dev(h, ctypes.c_void_p(dC.ptr), 3,
+                                                 ctypes.c_void_p(dP.ptr), None), "process_dev")
+    out = dP.to_host().reshape(3, N, N)
+    for b, c in enumerate(cvvs):
+        ref = oi.process(c, idx, a2f, otf, N, np_)
+        assert np.max(np.abs(out[b] - ref)) <= 2e-5 * np.max(np.abs(ref))
+    assert out[2][N // 2, N // 2] == pytest.approx(otf.sum(), rel=1e-5)  # Strehl 1
+    lib().moao_hip_intersample_destroy(h)
