@@ -1,6 +1,7 @@
 """Benchmark of the MOAO reconstructor hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload elt74] [--modes fp64,mixed]
+                    [--serve 32] [--serve-modes fp64[,mixed]] [--slots 4] [--no-cpu]
 
 One step = one R-solve of the workload: restore the resident inputs (device-to-device copies of
 the Cmm and Ctm generated once by matcov) and solve R = Ctm * Cmm^{-1} in place.  The inputs
@@ -11,6 +12,8 @@ value           R-solves/s of the headline path (mixed precision when built and 
 latency         p50 / p99 of the per-step wall time (each step synchronised)
 roofline        FP64 MFMA GEMM engine: algorithmic GEMM flops / summed GEMM kernel time
                 (HIP events on the launch stream) vs the 78.6 TFLOP/s FP64 peak
+serving         a Poisson stream of uneven systems (n up to 40k) through the serving engine:
+                throughput and p50/p99 latency from submit to R on the host
 cpu_baseline    the oracle's LAPACK path (scipy dpotrf + 2 dtrsm, OpenBLAS) timed on a bounded
                 leading sub-problem of the same matrices, scaled by flops to solves/s
 """
@@ -53,8 +56,8 @@ def cpu_baseline(dCmm0, dCtm0, n, m, target_s=12.0):
                        if p.get("internal_api") in ("openblas", "mkl", "blis")] or [1])
     except Exception:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    # size the sample for ~target_s seconds at ~30 GFLOP/s per thread
-    rate_guess = 30e9 * threads
+    # size the sample for ~target_s seconds at ~80 GFLOP/s per thread (OpenBLAS dgemm rate)
+    rate_guess = 80e9 * threads
     ns = int(min(n, (3.0 * target_s * rate_guess / (1 + 6 * m / n)) ** (1 / 3)))
     ns = max(512, ns - ns % 64)
     ms = max(1, int(round(m * ns / n)))
@@ -77,6 +80,95 @@ def cpu_baseline(dCmm0, dCtm0, n, m, target_s=12.0):
                       f"n={n}, m={m}"}
 
 
+def hbm_traffic(workload):
+    """HBM bytes per FP64 GEMM launch from the two calibrated rocprofv3 counter passes
+    (FETCH_SIZE, WRITE_SIZE) recorded for this workload, or None if none were recorded."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+    return rec["bytes_per_launch"] if rec else None
+
+
+def serving(nreq, mode, slots, seed=2024, util=0.8):
+    """Request stream of uneven systems with Poisson arrivals through the serving engine.
+
+    Each request draws n_wfs ~ U{4..9} measuring sensors (two thirds LGS, plus the truth
+    sensor) and Nssp ~ U{20..74} on the D=39 m telescope (n = 2 n_wfs Nvalid, redrawn while
+    n > 40,000), 10 layers with fresh
+    seeded Cn2 and r0 ~ U(0.08, 0.2) m.  Arrivals are Poisson at a rate that would keep one
+    GPU ~util busy by the flop model (solve flops at 40 TFLOP/s).  A request is one refine
+    step: Tomo from files, matcov Cmm and Ctm, solve, R back to the host."""
+    import threading
+    from moao_amd import systems
+    from moao_amd.lib import MOAO_FP64, MOAO_MIXED_FP32
+    from moao_amd.serve import Server
+    rng = np.random.default_rng(seed)
+    tmp = tempfile.mkdtemp(prefix="moao_serve_")
+    reqs = []
+    while len(reqs) < nreq:
+        nw = int(rng.integers(4, 10))
+        nssp = int(rng.integers(20, 75))
+        nv = systems.valid_subaps(nssp, 0.28)
+        n = 2 * nw * nv  # n_wfs sensors measure; the truth sensor is extra
+        if n > 40000:
+            continue
+        nlgs = max(1, (2 * nw + 2) // 3)
+        spec = systems.SystemSpec(diam=39.0, nssp=nssp, nlgs=nlgs, nngs=nw - nlgs, nlayer=10,
+                                  r0=float(rng.uniform(0.08, 0.2)),
+                                  seed=int(rng.integers(1 << 30)))
+        d = os.path.join(tmp, f"r{len(reqs)}")
+        os.makedirs(d)
+        s, atms = systems.make_system(d, spec)
+        m = 2 * nv
+        reqs.append((s, atms[0], n, m))
+    est = [solve_flops(n, m) / 40e12 for _, _, n, m in reqs]
+    rate = util / (sum(est) / len(est))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+    md = MOAO_FP64 if mode == "fp64" else MOAO_MIXED_FP32
+    results = [None] * nreq
+    with Server(slots=slots) as srv:
+        tickets = [None] * nreq
+        t0 = time.perf_counter()
+
+        def waiter(k):
+            results[k] = srv.wait(tickets[k], raise_on_error=False)
+
+        threads = []
+        for k, (s, a, n, m) in enumerate(reqs):
+            dt = arrivals[k] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets[k] = srv.submit(s, a, 0, md, tol=1e-9, max_iter=6, want_R=True,
+                                    n_m=(n, m))
+            th = threading.Thread(target=waiter, args=(k,))
+            th.start()
+            threads.append(th)
+        for th in threads:
+            th.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    ok = [r for r in results if r["status"] == 0]
+    lat = sorted(r["latency_s"] * 1e3 for r in ok)
+    q = sorted(r["queue_s"] * 1e3 for r in ok)
+    ns = [n for _, _, n, _ in reqs]
+
+    def pct(v, p):
+        return v[min(len(v) - 1, int(math.ceil(p * len(v))) - 1)] if v else None
+
+    return {"requests": nreq, "completed": len(ok), "mode": mode, "slots": slots,
+            "arrival_rate_per_s": rate, "wall_s": wall,
+            "throughput_solves_per_s": len(ok) / wall,
+            "latency_ms": {"p50": pct(lat, 0.5), "p99": pct(lat, 0.99),
+                           "max": lat[-1] if lat else None},
+            "queue_ms": {"p50": pct(q, 0.5), "p99": pct(q, 0.99)},
+            "peak_running": st["peak_running"], "peak_bytes": st["peak_bytes"],
+            "n": {"min": min(ns), "mean": sum(ns) / len(ns), "max": max(ns)},
+            "note": "latency = submit -> R on the host (queueing + Tomo + matcov + solve)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +180,13 @@ def main():
                     help="mixed path: relative residual ||Ctm - R Cmm||/||Ctm|| to stop at")
     ap.add_argument("--max-iter", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve", type=int, default=32,
+                    help="requests in the serving leg (0 = skip it)")
+    ap.add_argument("--serve-modes", default="fp64")
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the bench, run 3 plain device-to-device copies of this many "
+                         "MiB (the calibration kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -102,6 +201,16 @@ def main():
     from moao_amd.tomo import Tomo
 
     check(lib().moao_hip_init(0), "moao_hip_init")
+    if args.copy_probe_mib > 0:
+        nd = args.copy_probe_mib * (1 << 20) // 8
+        a, b = DeviceArray((nd,)), DeviceArray((nd,))
+        for _ in range(3):
+            b.copy_from(a)
+        synchronize()
+        print(f"[bench] copy probe: 3 x {args.copy_probe_mib} MiB device-to-device",
+              file=sys.stderr, flush=True)
+        a.free()
+        b.free()
     tmp = tempfile.mkdtemp(prefix="moao_bench_")
     sys_file, atms = systems.make_system(tmp, args.workload)
     tomo = Tomo(sys_file, atms[0])
@@ -194,7 +303,8 @@ def main():
                        "p99": 1e3 * lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]},
         "solve_tflops": flops / (ms_step * 1e-3) / 1e12,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "traffic": hbm_traffic(args.workload),
                      "kernel": "gemm_kernel<double> (all launches of the timed solves)",
                      "launches": g["count"], "avg_launch_ms": g["ms"] / max(1, g["count"])},
         "stages_ms_per_solve": {k: v["ms"] / args.steps for k, v in r["prof"].items()
@@ -213,6 +323,14 @@ def main():
                                             if "fp64" in results else None),
                         "stages_ms_per_solve": {k: v["ms"] / args.steps
                                                 for k, v in mr["prof"].items() if v["count"]}}
+    if args.serve > 0:
+        out["serving"] = []
+        for sm in args.serve_modes.split(","):
+            sv = serving(args.serve, sm, args.slots)
+            print(f"[bench] serving {sm}: {sv['throughput_solves_per_s']:.2f} solves/s, "
+                  f"p50 {sv['latency_ms']['p50']:.0f} ms, p99 {sv['latency_ms']['p99']:.0f} ms",
+                  file=sys.stderr, flush=True)
+            out["serving"].append(sv)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(dCmm0, dCtm0, n, m)
     print(json.dumps(out))

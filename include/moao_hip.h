@@ -191,6 +191,59 @@ int moao_hip_sgemm_dev(int transb, int M, int N, int K, float alpha, const float
 int moao_hip_profile_begin(void);
 int moao_hip_profile_end(double* ms, double* work, long* count);  /* arrays of 7 */
 
+/* ---------------------------------------------------------------- serving -----------------
+ * Continuous batching of reconstructor requests (new systems / atmospheric profiles) on one
+ * GPU.  The reference drives one system at a time from its host loop
+ * (lapack/moao_lapack.cpp:95-188: Tomo from files -> matcov Cmm, Ctm -> reconstructor); a
+ * request here is that refine step for one (sys-params, atm-params, target).  submit() reads
+ * the files and sizes the request (n, m, HBM footprint) on the caller's thread and returns at
+ * once; `slots` device streams each take the next request that fits the byte budget as soon as
+ * they are free, so requests arriving while others run are admitted as slots and memory free
+ * up, and their kernels overlap.  A request's kernels and inputs do not depend on the mix: its
+ * R is bit-identical alone and in a mix.                                                    */
+typedef struct moao_server moao_server;
+
+typedef struct moao_serve_config {
+    int slots;          /* concurrent requests on the device (0 -> 4, at most 16) */
+    size_t hbm_budget;  /* bytes of device memory requests may hold together (0 -> 85% free) */
+    int max_bypass;     /* times a queued request may be overtaken by smaller ones (0 -> 8) */
+} moao_serve_config;
+
+typedef struct moao_serve_request {
+    const char* sys_file;  /* reference-format sys-params file */
+    const char* atm_file;  /* reference-format atmospheric profile */
+    int target;            /* truth direction (Tomo_struct::setTarget) */
+    int mode;              /* MOAO_FP64 or MOAO_MIXED_FP32 */
+    double tol;            /* mixed: relative residual to refine to (0 -> 1e-11) */
+    int max_iter;          /* mixed: refinement steps allowed (0 -> 5) */
+    double* R;             /* host output m x n (column-major) or NULL; written before wait */
+    long ldR;
+} moao_serve_request;
+
+typedef struct moao_serve_result {
+    int status;            /* as the compute entries: 0, >0 dpotrf info, <0 MOAO_ERR_* */
+    int slot;
+    long n, m;
+    int iters;             /* mixed: refinement steps taken */
+    double resid;          /* mixed: final relative residual */
+    size_t footprint;      /* bytes of device memory the request held */
+    double t_submit, t_start, t_end;  /* seconds since the server was created */
+} moao_serve_result;
+
+typedef struct moao_serve_counters {
+    long queued, running, completed, peak_running;
+    size_t bytes_in_use, peak_bytes, budget;
+} moao_serve_counters;
+
+int moao_serve_create(const moao_serve_config* cfg, moao_server** out);
+/* MOAO_ERR_NOMEM when the request alone exceeds the budget; MOAO_ERR_IO on unreadable files. */
+int moao_serve_submit(moao_server* s, const moao_serve_request* req, long* ticket);
+/* Blocks until the request is done; returns its status and forgets the ticket. */
+int moao_serve_wait(moao_server* s, long ticket, moao_serve_result* out);
+int moao_serve_stats(moao_server* s, moao_serve_counters* out);
+/* Finishes every submitted request, then stops the slots. */
+void moao_serve_destroy(moao_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,7 +49,7 @@ struct Mfma<float> {
 // Loads are branch-free: addresses are clamped into the matrix and out-of-range values zeroed
 // by a select, so the compiler issues every load of a slice back to back.
 template <typename T, int TB>
-__global__ void __launch_bounds__(NT) gemm_kernel(GemmArgs<T> g) {
+__global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     using acc_t = typename Mfma<T>::acc_t;
     __shared__ T As[2][BK][BM + PAD];
     __shared__ T Bs[2][BK][BN + PAD];
@@ -74,19 +74,28 @@ __global__ void __launch_bounds__(NT) gemm_kernel(GemmArgs<T> g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    // Global -> register staging.  A (and B^T): lane runs down 128 contiguous rows of a column.
-    // Raw loads go to registers with a validity mask; the mask is applied when the values are
-    // written to LDS (after the slice's MFMAs), so no wait sits between the loads.
+    // Global -> register staging.  A (and B^T): a wave runs down 64 contiguous rows of one
+    // column; which column is wave-uniform (readfirstlane), so the column base is a scalar and
+    // the per-lane part is a row offset computed once per tile: no per-load address VALU.
+    // Raw loads go to registers with a validity mask applied at the LDS store (after the
+    // slice's MFMAs), so no wait sits between the loads.
     T ra[8], rb[8];
     unsigned ma = 0, mb = 0;
-    const int lrow = tid & 127, lk = tid >> 7;  // A / B^T pattern: 2 columns per pass
-    const int bk = tid & 15, bn = tid >> 4;     // B (K x N) pattern: 16 k per column
-    const unsigned ulda = (unsigned)s.lda, uldb = (unsigned)g.ldb;
+    const int lrow = tid & 127;
+    const int lk = __builtin_amdgcn_readfirstlane(tid >> 7);  // 0 for waves 0-1, 1 for 2-3
+    const int bk = tid & 15, bn = tid >> 4;                   // B (K x N) pattern
+    const long lda = s.lda, ldb = g.ldb;
     const int arow = m0 + lrow;
-    const T* __restrict__ Ap = s.A + min(arow, Mc);
-    const T* __restrict__ Bp = g.B;
+    const unsigned aoff = (unsigned)min(arow, Mc) * (unsigned)sizeof(T);  // bytes
     const bool arow_ok = arow < M;
-    auto off = [](int i, unsigned ld) { return (unsigned long)(unsigned)i * ld; };
+    const T* __restrict__ A = s.A;
+    const T* __restrict__ Bm = g.B;
+    // TB=0 / crossing-slice pattern: per-lane column offsets, computed once per tile
+    long bcol[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) bcol[r] = (long)min(n0 + bn + 16 * r, Nc) * ldb;
+    const unsigned bcoloff = (unsigned)min(n0 + lrow, Nc) * (unsigned)sizeof(T);  // bytes
+    const bool bcol_ok = n0 + lrow < N;
 
     int bpat = TB == 1 ? 1 : 0;  // layout of the slice held in rb
     auto load = [&](int k0) {
@@ -94,7 +103,8 @@ __global__ void __launch_bounds__(NT) gemm_kernel(GemmArgs<T> g) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int k = k0 + lk + 2 * r;
-            ra[r] = Ap[off(min(k, Kc), ulda)];
+            const T* col = A + (long)min(k, Kc) * lda;  // wave-uniform
+            ra[r] = *(const T*)((const char*)col + aoff);
             ma |= (unsigned)(arow_ok && k < K) << r;
         }
         int pat = TB == 1 ? 1 : 0;
@@ -105,30 +115,27 @@ __global__ void __launch_bounds__(NT) gemm_kernel(GemmArgs<T> g) {
         }
         bpat = pat == 1 ? 1 : 0;
         if (pat == 1) {
-            const int col = n0 + lrow;
-            const T* __restrict__ bp = Bp + min(col, Nc);
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
                 const int k = k0 + lk + 2 * r;
-                rb[r] = bp[off(min(k, Kc), uldb)];
-                mb |= (unsigned)(col < N && k < K) << r;
+                const T* col = Bm + (long)min(k, Kc) * ldb;  // wave-uniform
+                rb[r] = *(const T*)((const char*)col + bcoloff);
+                mb |= (unsigned)(bcol_ok && k < K) << r;
             }
         } else if (pat == 0) {
-            const int k = k0 + bk;
-            const T* __restrict__ bp = Bp + min(k, Kc);
+            const int k = k0 + bk, kc = min(k, Kc);
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                const int col = n0 + bn + 16 * r;
-                rb[r] = bp[off(min(col, Nc), uldb)];
-                mb |= (unsigned)(col < N && k < K) << r;
+                rb[r] = Bm[bcol[r] + kc];
+                mb |= (unsigned)(n0 + bn + 16 * r < N && k < K) << r;
             }
         } else {
             const int k = k0 + bk, kc = min(k, Kc);
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                const int col = n0 + bn + 16 * r, cc = min(col, Nc);
-                rb[r] = kc >= cc ? Bp[kc + off(cc, uldb)] : Bp[cc + off(kc, uldb)];
-                mb |= (unsigned)(col < N && k < K) << r;
+                const int cc = min(n0 + bn + 16 * r, Nc);
+                rb[r] = kc >= cc ? Bm[bcol[r] + kc] : Bm[cc + (long)kc * ldb];
+                mb |= (unsigned)(n0 + bn + 16 * r < N && k < K) << r;
             }
         }
     };

@@ -30,6 +30,7 @@
 
 #include "common.hpp"
 #include "profile.hpp"
+#include "stages.hpp"
 
 namespace moao {
 namespace {
@@ -240,7 +241,7 @@ size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 }  // namespace
 
 int matcov_tile_launch(double* d_data, int nrows, int ncols, int xoffset, int yoffset, int lda,
-                       const moao_tomo_view* t, int type_mat, hipStream_t stream) {
+                       const moao_tomo_view* t, int type_mat, hipStream_t stream, DevBuf* tables) {
     if (type_mat < 1 || type_mat > 4) {
         set_error("matcov_tile: unrecognized type_mat " + std::to_string(type_mat));
         return MOAO_ERR_ARG;
@@ -319,10 +320,15 @@ int matcov_tile_launch(double* d_data, int nrows, int ncols, int xoffset, int yo
     std::vector<unsigned char> host(total);
     for (auto& p : pieces)
         if (p.src && p.bytes) std::memcpy(host.data() + p.off, p.src, p.bytes);
-    Context& c = ctx();
-    std::lock_guard<std::mutex> g(c.mu);
-    if (int rc = c.matcov_tables.ensure(total)) return rc;
-    unsigned char* d = c.matcov_tables.as<unsigned char>();
+    // Tables of the library context are shared by every caller: hold the lock until the kernel
+    // that reads them has run.  A caller-owned buffer (one per serving slot) needs no lock.
+    std::unique_lock<std::mutex> g;
+    if (!tables) {
+        g = std::unique_lock<std::mutex>(ctx().mu);
+        tables = &ctx().matcov_tables;
+    }
+    if (int rc = tables->ensure(total)) return rc;
+    unsigned char* d = tables->as<unsigned char>();
     // The upload buffer is reused by the next call: order it on the stream and make the host
     // copy complete before returning (hipMemcpyAsync from pageable memory is staged).
     MOAO_HIP_TRY(hipMemcpyAsync(d, host.data(), total, hipMemcpyHostToDevice, stream));
@@ -391,7 +397,7 @@ int moao_hip_matcov_tile_dev(double* d_data, int nrows, int ncols, int xoffset, 
                              int lda, const moao_tomo_view* tomo, int type_mat, void* stream) {
     if (int rc = moao::ensure_init()) return rc;
     return moao::matcov_tile_launch(d_data, nrows, ncols, xoffset, yoffset, lda, tomo, type_mat,
-                                    moao::pick_stream(stream));
+                                    moao::pick_stream(stream), nullptr);
 }
 
 int moao_hip_matcov_tile(double* data, int nrows, int ncols, int xoffset, int yoffset, int lda,
@@ -412,9 +418,9 @@ int moao_hip_matcov_tile(double* data, int nrows, int ncols, int xoffset, int yo
     // start from the caller's contents so the copy back preserves them.
     if (type_mat == 1 || lda != nrows)
         MOAO_HIP_TRY(hipMemcpyAsync(d, data, span, hipMemcpyHostToDevice, s));
-    c.mu.unlock();
-    int rc = moao::matcov_tile_launch(d, nrows, ncols, xoffset, yoffset, lda, tomo, type_mat, s);
-    c.mu.lock();
+    // c.mu is held: hand the context's table buffer over explicitly
+    int rc = moao::matcov_tile_launch(d, nrows, ncols, xoffset, yoffset, lda, tomo, type_mat, s,
+                                      &c.matcov_tables);
     if (rc) return rc;
     MOAO_HIP_TRY(hipMemcpyAsync(data, d, span, hipMemcpyDeviceToHost, s));
     MOAO_HIP_TRY(hipStreamSynchronize(s));

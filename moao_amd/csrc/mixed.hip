@@ -28,6 +28,7 @@
 #include "leaf.hpp"
 #include "profile.hpp"
 #include "recon.hpp"
+#include "stages.hpp"
 
 namespace moao {
 namespace {
@@ -170,9 +171,41 @@ struct MixedWs {
 
 }  // namespace
 
+// Byte offsets of the workspace pieces (MixedWs order) and the total size.
+static size_t mixed_layout(long n, long m, int p, std::vector<size_t>* offs) {
+    const long xr = m + p;
+    const long q = std::max(p, 1);
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        if (offs) offs->push_back(off);
+        off += al(bytes);
+    };
+    take(sizeof(float) * n * n);
+    take(sizeof(float) * xr * n);
+    take(sizeof(float) * recon_invl_doubles(n));
+    take(sizeof(double) * m * n);
+    take(sizeof(double) * m * n);
+    take(sizeof(double) * n * q);
+    take(sizeof(double) * q * n);
+    take(sizeof(double) * m * q);
+    take(sizeof(double) * m * q);
+    take(sizeof(double) * q * q);
+    take(sizeof(double) * q * q);
+    take(sizeof(double) * kNormBlocks);
+    take(sizeof(int) * n);
+    take(sizeof(int) * 4);
+    return off;
+}
+
+size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank) {
+    return mixed_layout(n, m, lowrank_ranges(lowrank, n).p(), nullptr);
+}
+
 int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, double* Ctm, long ldCtm,
                             const moao_tomo_view* lowrank, double tol, int max_iter,
-                            double* resid_out, int* iters_out, hipStream_t s) {
+                            double* resid_out, int* iters_out, DevBuf& wsbuf, hipStream_t s,
+                            hipStream_t side) {
     if (n < 1 || m < 1 || ldCmm < n || ldCtm < m || !Cmm || !Ctm || max_iter < 0 ||
         n >= (1L << 31) || m >= (1L << 31)) {
         set_error("reconstructor_mixed: invalid arguments");
@@ -182,28 +215,10 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
     const Ranges rg = lowrank_ranges(lowrank, n);
     const int p = rg.p();
     const long xr = m + p;  // rows of the FP32 right-hand-side block
-    // workspace layout
-    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    size_t off = 0;
     std::vector<size_t> offs;
-    auto take = [&](size_t bytes) { offs.push_back(off); off += al(bytes); };
-    take(sizeof(float) * n * n);
-    take(sizeof(float) * xr * n);
-    take(sizeof(float) * recon_invl_doubles(n));
-    take(sizeof(double) * m * n);
-    take(sizeof(double) * m * n);
-    take(sizeof(double) * n * std::max(p, 1));
-    take(sizeof(double) * std::max(p, 1) * n);
-    take(sizeof(double) * m * std::max(p, 1));
-    take(sizeof(double) * m * std::max(p, 1));
-    take(sizeof(double) * std::max(p * p, 1));
-    take(sizeof(double) * std::max(p * p, 1));
-    take(sizeof(double) * kNormBlocks);
-    take(sizeof(int) * n);
-    take(sizeof(int) * 4);
-    Context& c = ctx();
-    if (int rc = c.mixed_ws.ensure(off)) return rc;
-    char* b = c.mixed_ws.as<char>();
+    const size_t total = mixed_layout(n, m, p, &offs);
+    if (int rc = wsbuf.ensure(total)) return rc;
+    char* b = wsbuf.as<char>();
     MixedWs w{(float*)(b + offs[0]), (float*)(b + offs[1]), (float*)(b + offs[2]),
               (double*)(b + offs[3]), (double*)(b + offs[4]), (double*)(b + offs[5]),
               (double*)(b + offs[6]), (double*)(b + offs[7]), (double*)(b + offs[8]),
@@ -266,7 +281,7 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
 
     // --- 2. FP32 factorization with [Ctm; U^T] riding along, second solve
     std::vector<RowGroup<float>> all{{w.X32, xr, (int)xr}};
-    if (int rc = potrf_tall<float>(n, w.C0, n, all, w.invL, w.info, s)) return rc;
+    if (int rc = potrf_tall<float>(n, w.C0, n, all, w.invL, w.info, s, side)) return rc;
     int hinfo = 0;
     MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
     MOAO_HIP_TRY(hipStreamSynchronize(s));
@@ -377,7 +392,8 @@ int moao_hip_reconstructor_mixed_dev(long nmeas, long nmeasts, const double* dCm
     }
     std::lock_guard<std::mutex> g(moao::ctx().mu);
     return moao::reconstructor_mixed_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, lowrank, tol,
-                                         max_iter, resid, iters, moao::pick_stream(stream));
+                                         max_iter, resid, iters, moao::ctx().mixed_ws,
+                                         moao::pick_stream(stream), moao::ctx().side);
 }
 
 }  // extern "C"

@@ -43,6 +43,35 @@ class TomoView(ctypes.Structure):
                 ("noiseLGSyy", _c_double_p), ("noiseLGSxy", _c_double_p)]
 
 
+class ServeConfig(ctypes.Structure):
+    """ctypes mirror of moao_serve_config."""
+    _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("max_bypass", ctypes.c_int)]
+
+
+class ServeRequest(ctypes.Structure):
+    """ctypes mirror of moao_serve_request."""
+    _fields_ = [("sys_file", ctypes.c_char_p), ("atm_file", ctypes.c_char_p),
+                ("target", ctypes.c_int), ("mode", ctypes.c_int), ("tol", ctypes.c_double),
+                ("max_iter", ctypes.c_int), ("R", _c_double_p), ("ldR", ctypes.c_long)]
+
+
+class ServeResult(ctypes.Structure):
+    """ctypes mirror of moao_serve_result."""
+    _fields_ = [("status", ctypes.c_int), ("slot", ctypes.c_int), ("n", ctypes.c_long),
+                ("m", ctypes.c_long), ("iters", ctypes.c_int), ("resid", ctypes.c_double),
+                ("footprint", ctypes.c_size_t), ("t_submit", ctypes.c_double),
+                ("t_start", ctypes.c_double), ("t_end", ctypes.c_double)]
+
+
+class ServeCounters(ctypes.Structure):
+    """ctypes mirror of moao_serve_counters."""
+    _fields_ = [("queued", ctypes.c_long), ("running", ctypes.c_long),
+                ("completed", ctypes.c_long), ("peak_running", ctypes.c_long),
+                ("bytes_in_use", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t),
+                ("budget", ctypes.c_size_t)]
+
+
 _lib = None
 
 # name -> (restype, argtypes); the exported C-ABI
@@ -118,6 +147,14 @@ _SIGS = {
     "moao_hip_intersample_destroy": (None, [ctypes.c_void_p]),
     "moao_hip_profile_begin": (ctypes.c_int, []),
     "moao_hip_profile_end": (ctypes.c_int, [_c_double_p, _c_double_p, _c_long_p]),
+    "moao_serve_create": (ctypes.c_int, [ctypes.POINTER(ServeConfig),
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "moao_serve_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeRequest),
+                                         _c_long_p]),
+    "moao_serve_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_long,
+                                       ctypes.POINTER(ServeResult)]),
+    "moao_serve_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeCounters)]),
+    "moao_serve_destroy": (None, [ctypes.c_void_p]),
 }
 
 PROF_CLASSES = ["gemm_f64", "gemm_f32", "leaf", "matcov", "convert", "fft", "other"]
