@@ -99,8 +99,8 @@ def serving(nreq, mode, slots, seed=2024, util=0.8):
     sensor) and Nssp ~ U{20..74} on the D=39 m telescope (n = 2 n_wfs Nvalid, redrawn while
     n > 40,000), 10 layers with fresh
     seeded Cn2 and r0 ~ U(0.08, 0.2) m.  Arrivals are Poisson at a rate that would keep one
-    GPU ~util busy by the flop model (solve flops at 40 TFLOP/s).  A request is one refine
-    step: Tomo from files, matcov Cmm and Ctm, solve, R back to the host."""
+    GPU ~util busy by a service-time model (solve flops, matcov elements, R bytes).  A request
+    is one refine step: Tomo from files, matcov Cmm and Ctm, solve, R back to the host."""
     import threading
     from moao_amd import systems
     from moao_amd.lib import MOAO_FP64, MOAO_MIXED_FP32
@@ -124,7 +124,10 @@ def serving(nreq, mode, slots, seed=2024, util=0.8):
         s, atms = systems.make_system(d, spec)
         m = 2 * nv
         reqs.append((s, atms[0], n, m))
-    est = [solve_flops(n, m) / 40e12 for _, _, n, m in reqs]
+    # service-time model of one request on an otherwise idle GPU: solve at 40 TFLOP/s,
+    # matcov at 2.3e9 elements/s (10 layers), R to pageable host memory at 20 GB/s
+    est = [solve_flops(n, m) / 40e12 + (n * (n + 1) / 2 + m * n) / 2.3e9 + 8.0 * m * n / 20e9
+           for _, _, n, m in reqs]
     rate = util / (sum(est) / len(est))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
     md = MOAO_FP64 if mode == "fp64" else MOAO_MIXED_FP32

@@ -38,7 +38,6 @@ struct DevBuf {
 struct Context {
     int device = -1;
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;  // second stream for work that overlaps the main chain
     DevBuf matcov_tables;   // per-call tomo upload for matcov
     DevBuf host_stage;      // device copy of a host matrix for the host-pointer entry points
     DevBuf host_stage2;

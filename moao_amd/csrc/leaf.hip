@@ -44,10 +44,9 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
     if (*(volatile int*)info != 0) return;  // an earlier leaf failed: R is not computed
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) fail = 0;
-    for (int e = tid; e < nb * nb; e += NTL) {
-        const int r = e % nb, c = e / nb;
-        S_(r, c) = (r >= c) ? (double)A[r + (long)c * lda] : 0.0;
-    }
+    // a wave per column, lanes down the rows (no runtime division in the index math)
+    for (int c = wave; c < nb; c += NTL / 64)
+        for (int r = lane; r < nb; r += 64) S_(r, c) = (r >= c) ? (double)A[r + (long)c * lda] : 0.0;
     __syncthreads();
 
     // ------------------------------------------------------------------ factorization
@@ -130,10 +129,8 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
         __syncthreads();
     }
     // L back to global (lower part only).
-    for (int e = tid; e < nb * nb; e += NTL) {
-        const int r = e % nb, c = e / nb;
-        if (r >= c) A[r + (long)c * lda] = (TS)S_(r, c);
-    }
+    for (int c = wave; c < nb; c += NTL / 64)
+        for (int r = c + lane; r < nb; r += 64) A[r + (long)c * lda] = (TS)S_(r, c);
 
     // ------------------------------------------------------------------ inverse
     const int nblocks = (nb + kB - 1) / kB;
@@ -160,31 +157,30 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
             for (int r = 0; r < kB; ++r) D[r * kB + c] = x[r];  // D[r][c]
         }
         // T = inv(L)[below, below] * L[below, c0:c0+bs]   (inv(L) trailing part already in S)
-        for (int e = tid; e < below * bs; e += NTL) {
-            const int rr = e % below, c = e / below;
-            const int r = c0 + bs + rr;
-            double v = 0.0;
-            for (int k = c0 + bs; k <= r; ++k) v += S_(r, k) * S_(k, c0 + c);
-            T[c * (kLeaf - kB) + rr] = v;
-        }
+        // thread -> (row rr = tid/16 + 16 i, column c = tid%16): no runtime division
+        const int tc = tid & (kB - 1), tr = tid / kB;
+        if (tc < bs)
+            for (int rr = tr; rr < below; rr += NTL / kB) {
+                const int r = c0 + bs + rr;
+                double v = 0.0;
+                for (int k = c0 + bs; k <= r; ++k) v += S_(r, k) * S_(k, c0 + tc);
+                T[tc * (kLeaf - kB) + rr] = v;
+            }
         __syncthreads();
         // column block jb of inv(L): rows below = -T * D; diagonal block = D
-        for (int e = tid; e < below * bs; e += NTL) {
-            const int rr = e % below, c = e / below;
-            double v = 0.0;
-            for (int k = c; k < bs; ++k) v += T[k * (kLeaf - kB) + rr] * D[k * kB + c];
-            S_(c0 + bs + rr, c0 + c) = -v;
-        }
-        for (int e = tid; e < bs * bs; e += NTL) {
-            const int r = e % bs, c = e / bs;
-            S_(c0 + r, c0 + c) = r >= c ? D[r * kB + c] : 0.0;
+        if (tc < bs) {
+            for (int rr = tr; rr < below; rr += NTL / kB) {
+                double v = 0.0;
+                for (int k = tc; k < bs; ++k) v += T[k * (kLeaf - kB) + rr] * D[k * kB + tc];
+                S_(c0 + bs + rr, c0 + tc) = -v;
+            }
+            if (tr < bs) S_(c0 + tr, c0 + tc) = tr >= tc ? D[tr * kB + tc] : 0.0;
         }
         __syncthreads();
     }
-    for (int e = tid; e < kLeaf * kLeaf; e += NTL) {
-        const int r = e % kLeaf, c = e / kLeaf;
-        inv[r + (long)c * kLeaf] = (TS)((r < nb && c < nb && r >= c) ? S_(r, c) : 0.0);
-    }
+    for (int c = wave; c < kLeaf; c += NTL / 64)
+        for (int r = lane; r < kLeaf; r += 64)
+            inv[r + (long)c * kLeaf] = (TS)((r < nb && c < nb && r >= c) ? S_(r, c) : 0.0);
 }
 
 #undef S_

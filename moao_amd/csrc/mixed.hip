@@ -204,8 +204,7 @@ size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank) {
 
 int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, double* Ctm, long ldCtm,
                             const moao_tomo_view* lowrank, double tol, int max_iter,
-                            double* resid_out, int* iters_out, DevBuf& wsbuf, hipStream_t s,
-                            hipStream_t side) {
+                            double* resid_out, int* iters_out, DevBuf& wsbuf, hipStream_t s) {
     if (n < 1 || m < 1 || ldCmm < n || ldCtm < m || !Cmm || !Ctm || max_iter < 0 ||
         n >= (1L << 31) || m >= (1L << 31)) {
         set_error("reconstructor_mixed: invalid arguments");
@@ -281,7 +280,7 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
 
     // --- 2. FP32 factorization with [Ctm; U^T] riding along, second solve
     std::vector<RowGroup<float>> all{{w.X32, xr, (int)xr}};
-    if (int rc = potrf_tall<float>(n, w.C0, n, all, w.invL, w.info, s, side)) return rc;
+    if (int rc = potrf_tall<float>(n, w.C0, n, all, w.invL, w.info, s)) return rc;
     int hinfo = 0;
     MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
     MOAO_HIP_TRY(hipStreamSynchronize(s));
@@ -393,7 +392,7 @@ int moao_hip_reconstructor_mixed_dev(long nmeas, long nmeasts, const double* dCm
     std::lock_guard<std::mutex> g(moao::ctx().mu);
     return moao::reconstructor_mixed_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, lowrank, tol,
                                          max_iter, resid, iters, moao::ctx().mixed_ws,
-                                         moao::pick_stream(stream), moao::ctx().side);
+                                         moao::pick_stream(stream));
 }
 
 }  // extern "C"

@@ -36,31 +36,6 @@ long split(long nn) {
     return n1;
 }
 
-// Events that order the side stream after the main one; recorded and waited in turn (a wait
-// captures the record made before it, so an event can be re-recorded afterwards).
-struct EventRing {
-    std::vector<hipEvent_t> ev;
-    size_t next = 0;
-    int init(size_t k) {
-        for (size_t i = 0; i < k; ++i) {
-            hipEvent_t e;
-            MOAO_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            ev.push_back(e);
-        }
-        return MOAO_OK;
-    }
-    ~EventRing() {
-        for (auto e : ev) (void)hipEventDestroy(e);
-    }
-    // b waits for everything enqueued on a so far
-    int order(hipStream_t a, hipStream_t b) {
-        hipEvent_t e = ev[next++ % ev.size()];
-        MOAO_HIP_TRY(hipEventRecord(e, a));
-        MOAO_HIP_TRY(hipStreamWaitEvent(b, e, 0));
-        return MOAO_OK;
-    }
-};
-
 template <typename T>
 struct Tall {
     long n;
@@ -69,13 +44,9 @@ struct Tall {
     const std::vector<RowGroup<T>>* groups;
     T* invL;
     int* info;
-    hipStream_t s;     // the n x n factorization (the critical path)
-    hipStream_t side;  // row-group updates (== s when not split)
-    EventRing* ring;   // non-null when side != s
+    hipStream_t s;
     T* a(long r, long c) const { return A + r + c * lda; }
     T* inv(long j0) const { return invL + (j0 / kLeaf) * (long)kLeaf * kLeaf; }
-    // row-group work that reads L written so far on s goes to `side` after this
-    int gate() const { return ring ? ring->order(s, side) : MOAO_OK; }
 };
 
 // Launch one GEMM over [optional lead segment] + every row group, kMaxSeg segments at a time.
@@ -123,20 +94,14 @@ int potrf_rec(const Tall<T>& p, long j0, long nn) {
         proto.transb = 1;
         proto.alpha = T(1);
         proto.beta = T(0);
-        auto lead = [&] {
-            return GemmSeg<T>{p.a(j0 + nn, j0), p.a(j0 + nn, j0), p.lda, p.lda, (int)below};
-        };
-        auto grp = [&](const RowGroup<T>& g) {
-            T* c = g.p + j0 * g.ld;
-            return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
-        };
-        if (!p.ring)
-            return over_groups<T>(G, proto, below > 0, lead, grp, p.s);
-        if (int rc = p.gate()) return rc;
-        if (below > 0)
-            if (int rc = over_groups<T>({}, proto, true, lead, grp, p.s)) return rc;
-        auto none = [] { return GemmSeg<T>{}; };
-        return over_groups<T>(G, proto, false, none, grp, p.side);
+        return over_groups<T>(
+            G, proto, below > 0,
+            [&] { return GemmSeg<T>{p.a(j0 + nn, j0), p.a(j0 + nn, j0), p.lda, p.lda, (int)below}; },
+            [&](const RowGroup<T>& g) {
+                T* c = g.p + j0 * g.ld;
+                return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
+            },
+            p.s);
     }
     const long n1 = split(nn);
     if (int rc = potrf_rec(p, j0, n1)) return rc;
@@ -152,21 +117,14 @@ int potrf_rec(const Tall<T>& p, long j0, long nn) {
     proto.tri_off = 0;
     proto.alpha = T(-1);
     proto.beta = T(1);
-    auto lead = [&] {
-        return GemmSeg<T>{p.a(r0, j0), p.a(r0, r0), p.lda, p.lda, (int)(p.n - r0)};
-    };
-    auto grp = [&](const RowGroup<T>& g) {
-        return GemmSeg<T>{g.p + j0 * g.ld, g.p + r0 * g.ld, g.ld, g.ld, g.rows};
-    };
-    if (!p.ring) {
-        if (int rc = over_groups<T>(G, proto, true, lead, grp, p.s)) return rc;
-    } else {
-        // the group update reads L[r0:, j0:r0], final once the left half is done on s
-        if (int rc = p.gate()) return rc;
-        if (int rc = over_groups<T>({}, proto, true, lead, grp, p.s)) return rc;
-        auto none = [] { return GemmSeg<T>{}; };
-        if (int rc = over_groups<T>(G, proto, false, none, grp, p.side)) return rc;
-    }
+    int rc = over_groups<T>(
+        G, proto, true,
+        [&] { return GemmSeg<T>{p.a(r0, j0), p.a(r0, r0), p.lda, p.lda, (int)(p.n - r0)}; },
+        [&](const RowGroup<T>& g) {
+            return GemmSeg<T>{g.p + j0 * g.ld, g.p + r0 * g.ld, g.ld, g.ld, g.rows};
+        },
+        p.s);
+    if (rc) return rc;
     return potrf_rec(p, r0, nn - n1);
 }
 
@@ -258,41 +216,30 @@ int trsmT_rec(const Tall<T>& p, long j0, long nn) {
 
 template <typename T>
 int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
-               int* info, hipStream_t s, hipStream_t side) {
-    bool split = side && side != s;
-    for (const auto& g : groups) split = split && g.rows > 0;
-    EventRing ring;
-    if (split)
-        if (int rc = ring.init(8)) return rc;
-    Tall<T> p{n, A, lda, &groups, invL, info, s, split ? side : s, split ? &ring : nullptr};
+               int* info, hipStream_t s) {
+    Tall<T> p{n, A, lda, &groups, invL, info, s};
     MOAO_HIP_TRY(hipMemsetAsync(info, 0, sizeof(int), s));
-    int rc = potrf_rec(p, 0, n);
-    // join: the caller's stream sees the row groups complete (also on an error return)
-    if (split) {
-        const int rj = ring.order(side, s);
-        if (!rc) rc = rj;
-    }
-    return rc;
+    return potrf_rec(p, 0, n);
 }
 
 template <typename T>
 int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
                              const std::vector<RowGroup<T>>& groups, hipStream_t s) {
-    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, s, s, nullptr};
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, s};
     return trsmN_rec(p, 0, n);
 }
 
 template <typename T>
 int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
                            const std::vector<RowGroup<T>>& groups, hipStream_t s) {
-    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, s, s, nullptr};
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, s};
     return trsmT_rec(p, 0, n);
 }
 
 template int potrf_tall<double>(long, double*, long, const std::vector<RowGroup<double>>&,
-                                double*, int*, hipStream_t, hipStream_t);
+                                double*, int*, hipStream_t);
 template int potrf_tall<float>(long, float*, long, const std::vector<RowGroup<float>>&, float*,
-                               int*, hipStream_t, hipStream_t);
+                               int*, hipStream_t);
 template int trsm_right_lower_notrans<double>(long, const double*, long, const double*,
                                               const std::vector<RowGroup<double>>&, hipStream_t);
 template int trsm_right_lower_notrans<float>(long, const float*, long, const float*,
@@ -303,8 +250,7 @@ template int trsm_right_lower_trans<float>(long, const float*, long, const float
                                            const std::vector<RowGroup<float>>&, hipStream_t);
 
 int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm, long ldCtm,
-                          long ngal, double* dinvL, int* dinfo, hipStream_t stream,
-                          hipStream_t side) {
+                          long ngal, double* dinvL, int* dinfo, hipStream_t stream) {
     if (n < 1 || m < 0 || ngal < 1 || ldCmm < n || (m > 0 && ldCtm < m) || !dCmm || !dinvL ||
         !dinfo || n >= (1L << 31) || m >= (1L << 31)) {
         set_error("reconstructor: invalid arguments");
@@ -313,8 +259,7 @@ int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm
     std::vector<RowGroup<double>> groups;
     if (m > 0)
         for (long g = 0; g < ngal; ++g) groups.push_back({dCtm + g * n * m, ldCtm, (int)m});
-    if (int rc = potrf_tall<double>(n, dCmm, ldCmm, groups, dinvL, dinfo, stream, side))
-        return rc;
+    if (int rc = potrf_tall<double>(n, dCmm, ldCmm, groups, dinvL, dinfo, stream)) return rc;
     int hinfo = 0;
     MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, stream));
     MOAO_HIP_TRY(hipStreamSynchronize(stream));
@@ -351,7 +296,7 @@ int moao_hip_reconstructor_dev(long nmeas, long nmeasts, double* dCmm, long ldCm
     std::lock_guard<std::mutex> g(moao::ctx().mu);
     if (int rc = recon_workspace(nmeas, &invL, &info)) return rc;
     return moao::reconstructor_f64_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, ngal, invL,
-                                       info, moao::pick_stream(stream), moao::ctx().side);
+                                       info, moao::pick_stream(stream));
 }
 
 int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, double* Ctm,
@@ -380,8 +325,7 @@ int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, do
     double* invL;
     int* info;
     if (int rc = recon_workspace(nmeas, &invL, &info)) return rc;
-    int rc = moao::reconstructor_f64_dev(nmeas, nmeasts, dA, ldCmm, dB, ldCtm, ngal, invL, info, s,
-                                         c.side);
+    int rc = moao::reconstructor_f64_dev(nmeas, nmeasts, dA, ldCmm, dB, ldCtm, ngal, invL, info, s);
     if (rc < 0) return rc;
     // dpotrf overwrites Cmm with its factor even when it stops early; R only on success.
     MOAO_HIP_TRY(hipMemcpyAsync(Cmm, dA, a_bytes, hipMemcpyDeviceToHost, s));

@@ -21,12 +21,9 @@ struct RowGroup {
 // Recursive factorization of the n x n SPD block (lower, in place) with the row groups
 // solved by L^T on the way (first right-side solve fused).  invL receives the inverse of every
 // kLeaf leaf (ceil(n/kLeaf)*kLeaf*kLeaf elements); *info the dpotrf status (device int).
-// With a `side` stream the row-group updates run there, each gated by an event on s after
-// the L columns it reads, so they fill the device while s is on the latency-bound leaves;
-// on return s is ordered after all of it.
 template <typename T>
 int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
-               int* info, hipStream_t s, hipStream_t side = nullptr);
+               int* info, hipStream_t s);
 // Y * L = X for every group (second right-side solve), with L / invL from potrf_tall.
 template <typename T>
 int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
@@ -40,10 +37,8 @@ int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
 // target blocks Ctm_g = dCtm + g*n*m (m x n, leading dimension ldCtm) is overwritten by R_g.
 // On a non-positive pivot the return value is its 1-based order and the Ctm blocks hold
 // partial results.  dinvL: workspace of recon_invl_elems(n) doubles; dinfo: one device int.
-// side: optional second stream for the Ctm rows of the factorization (see potrf_tall).
 int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm, long ldCtm,
-                          long ngal, double* dinvL, int* dinfo, hipStream_t stream,
-                          hipStream_t side = nullptr);
+                          long ngal, double* dinvL, int* dinfo, hipStream_t stream);
 
 // Number of elements of the inverse-leaf workspace.
 size_t recon_invl_doubles(long n);

@@ -63,7 +63,6 @@ static int init_device(int device) {
     MOAO_HIP_TRY(hipSetDevice(device));
     // Default-priority streams only: mixing stream priorities can stall work indefinitely.
     MOAO_HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
-    MOAO_HIP_TRY(hipStreamCreateWithFlags(&g_ctx.side, hipStreamNonBlocking));
     g_ctx.device = device;
     return MOAO_OK;
 }
@@ -226,7 +225,6 @@ int moao_hip_finalize(void) {
     if (ctx().device < 0) return MOAO_OK;
     moao::Context& c = ctx();
     (void)hipStreamSynchronize(c.stream);
-    (void)hipStreamSynchronize(c.side);
     c.matcov_tables.release();
     c.host_stage.release();
     c.host_stage2.release();

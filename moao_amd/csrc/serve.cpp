@@ -82,8 +82,7 @@ struct Request {
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;  // the factorization's right-hand-side rows
-    DevBuf tables;               // matcov upload tables (small, kept)
+    DevBuf tables;  // matcov upload tables (small, kept)
 };
 
 }  // namespace
@@ -141,7 +140,7 @@ int moao_server::run(moao::Request& r, moao::Slot& sl) {
         if (r.mode == MOAO_FP64) {
             double* invL = reinterpret_cast<double*>(scratch);
             int* info = reinterpret_cast<int*>(scratch + al256(recon_invl_doubles(n) * 8));
-            rc = reconstructor_f64_dev(n, m, Cmm, n, Ctm, m, 1, invL, info, s, sl.side);
+            rc = reconstructor_f64_dev(n, m, Cmm, n, Ctm, m, 1, invL, info, s);
         } else {
             DevBuf ws;  // wraps the request block: ensure() finds it large enough
             ws.ptr = scratch;
@@ -149,7 +148,7 @@ int moao_server::run(moao::Request& r, moao::Slot& sl) {
             double resid = 0;
             int iters = 0;
             rc = reconstructor_mixed_dev(n, m, Cmm, n, Ctm, m, &r.view, r.tol, r.max_iter, &resid,
-                                         &iters, ws, s, sl.side);
+                                         &iters, ws, s);
             r.res.resid = resid;
             r.res.iters = iters;
         }
@@ -231,12 +230,9 @@ int moao_serve_create(const moao_serve_config* cfg, moao_server** out) {
     if (cfg && cfg->max_bypass > 0) s->max_bypass = cfg->max_bypass;
     s->slots.resize(nslots);
     for (auto& sl : s->slots) {
-        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&sl.side, hipStreamNonBlocking) != hipSuccess) {
-            for (auto& o : s->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) {
+            for (auto& o : s->slots)
                 if (o.stream) (void)hipStreamDestroy(o.stream);
-                if (o.side) (void)hipStreamDestroy(o.side);
-            }
             delete s;
             moao::set_error("serve: stream creation failed");
             return MOAO_ERR_HIP;
@@ -343,7 +339,6 @@ void moao_serve_destroy(moao_server* s) {
     for (auto& sl : s->slots) {
         sl.tables.release();
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
-        if (sl.side) (void)hipStreamDestroy(sl.side);
     }
     delete s;
 }

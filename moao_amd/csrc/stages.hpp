@@ -20,10 +20,8 @@ size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank);
 
 // Mixed-precision reconstructor (FP32 factorization of C0 = Cmm - lgs_cst U U^T, Woodbury,
 // FP64 refinement to tol).  Cmm is read only; Ctm is overwritten by R.  ws grows as needed.
-// side: optional second stream for the right-hand-side rows of the factorization.
 int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, double* Ctm, long ldCtm,
                             const moao_tomo_view* lowrank, double tol, int max_iter,
-                            double* resid_out, int* iters_out, DevBuf& ws, hipStream_t s,
-                            hipStream_t side = nullptr);
+                            double* resid_out, int* iters_out, DevBuf& ws, hipStream_t s);
 
 }  // namespace moao
