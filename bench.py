@@ -92,6 +92,100 @@ def hbm_traffic(workload):
     return rec["bytes_per_launch"] if rec else None
 
 
+def cee_cvv_flops(n, m, nact):
+    """syr2k + symm + gemm + symm + gemm of lapack/compute/cee_cvv.cpp:55-117."""
+    return 2.0 * m * m * n + 2.0 * m * n * n + 2.0 * m * m * n + 2.0 * nact * m * m \
+        + 2.0 * nact * nact * m
+
+
+def stages(tomo, n, m, dCmm0, dCtm0, dR, ndir=8, N=2048, dm_nssp=79, reps=2):
+    """Observation-loop stages on the same system: Cee/Cvv (nact ~ 5000 on an 80 x 80 DM grid)
+    and the intersample PSF (N x N FP32 FFTs, ndir directions batched).  Inputs resident in
+    HBM; each timed over `reps` calls after one warm-up."""
+    import ctypes
+    from moao_amd import psf_inputs, systems
+    from moao_amd.device import DeviceArray, Profile, synchronize
+    from moao_amd.lib import check, lib
+    from moao_amd.matcov import CTT, matcov_tile_dev
+    vp = ctypes.c_void_p
+    dm = psf_inputs.make_dm(dm_nssp)
+    nact = dm.nact
+    out = {}
+    # ---- Cee/Cvv
+    dDx = DeviceArray.from_host(systems.dx_matrix(nact, m, n))
+    dCtt = DeviceArray((m, m))
+    matcov_tile_dev(dCtt.ptr, m, m, 0, 0, m, tomo, CTT)
+    dCpp, dCpm = DeviceArray((m, m)), DeviceArray((m, n))
+    dCee, dCvv, dTmp = DeviceArray((m, m)), DeviceArray((nact, nact)), DeviceArray((nact, m))
+    ms = []
+    for it in range(reps + 1):
+        dCpp.copy_from(dCtt)   # Cpp and Cpm are clobbered by each call, as in the reference
+        dCpm.copy_from(dCtm0)
+        synchronize()
+        with Profile() as prof:
+            t0 = time.perf_counter()
+            check(lib().moao_hip_cee_cvv_dev(n, m, nact, vp(dCmm0.ptr), n, vp(dCpp.ptr), m,
+                                             vp(dCpm.ptr), m, vp(dR.ptr), m, vp(dDx.ptr), nact,
+                                             vp(dCee.ptr), m, vp(dCvv.ptr), nact, vp(dTmp.ptr),
+                                             nact, None), "moao_hip_cee_cvv_dev")
+            synchronize()
+            dt = time.perf_counter() - t0
+        if it:
+            ms.append(dt * 1e3)
+    g = prof.result["gemm_f64"]
+    fl = cee_cvv_flops(n, m, nact)
+    out["cee_cvv"] = {"n": n, "m": m, "nact": nact, "ms": min(ms), "flops": fl,
+                      "tflops": fl / (min(ms) * 1e-3) / 1e12,
+                      "gemm_tflops": g["work"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] else None,
+                      "peak": PEAK_FP64 / 1e12, "bound": "mfma (FP64)"}
+    for d in (dDx, dCtt, dCpp, dCpm, dCee, dTmp):
+        d.free()
+    # ---- intersample PSF on the Cvv just computed, ndir directions batched
+    idx = psf_inputs.make_idx(dm)
+    diam = tomo.view().diam
+    a2f = psf_inputs.make_abs2fi(N, diam, dm)
+    otf = psf_inputs.make_otf_tel(N, diam, 0.28)
+    h = ctypes.c_void_p()
+    lp = ctypes.POINTER(ctypes.c_long)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(lib().moao_hip_intersample_create(idx.size, dm.ndiam, N, idx.ctypes.data_as(lp),
+                                            a2f.ctypes.data_as(dp), otf.ctypes.data_as(dp), 0,
+                                            ctypes.byref(h)), "intersample_create")
+    del idx
+    dC = DeviceArray((ndir * nact * nact,))
+    for k in range(ndir):
+        lib().moao_hip_memcpy_d2d(vp(dC.ptr + k * nact * nact * 8), vp(dCvv.ptr),
+                                  nact * nact * 8)
+    dP = DeviceArray((ndir * N * N,))
+    ms = []
+    for it in range(reps + 1):
+        synchronize()
+        with Profile() as prof:
+            t0 = time.perf_counter()
+            check(lib().moao_hip_intersample_process_dev(h, vp(dC.ptr), ndir, vp(dP.ptr), None),
+                  "intersample_process_dev")
+            synchronize()
+            dt = time.perf_counter() - t0
+        if it:
+            ms.append(dt * 1e3)
+    f = prof.result["fft"]
+    # algorithmic HBM bytes of one direction: Cvv gather (8 B per entry + 4 B index) plus
+    # 3 complex 2-D FFTs of N x N FP32 (each 2 row passes + 2 transposes, read + write 8 B)
+    # plus the pointwise passes (~6 N^2 x 8 B) and the FP64 PSF write
+    byt = ndir * (12.0 * nact * nact + 3 * 4 * 2 * 8.0 * N * N + 6 * 8.0 * N * N + 8.0 * N * N)
+    psf = dP.to_host()[:N * N].reshape(N, N)
+    out["psf"] = {"N": N, "directions": ndir, "nact": nact, "precision": "fp32 FFT",
+                  "ms": min(ms), "ms_per_direction": min(ms) / ndir,
+                  "fft_kernel_ms": f["ms"], "achieved_GBps": byt / (min(ms) * 1e-3) / 1e9,
+                  "peak_GBps": 8000.0, "bound": "hbm",
+                  "strehl_dir0": float(psf[N // 2, N // 2])}
+    lib().moao_hip_intersample_destroy(h)
+    dC.free()
+    dP.free()
+    dCvv.free()
+    return out
+
+
 def serving(nreq, mode, slots, seed=2024, util=0.8):
     """Request stream of uneven systems with Poisson arrivals through the serving engine.
 
@@ -187,6 +281,8 @@ def main():
                     help="requests in the serving leg (0 = skip it)")
     ap.add_argument("--serve-modes", default="fp64")
     ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--no-stages", action="store_true",
+                    help="skip the Cee/Cvv and PSF stage measurements")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the bench, run 3 plain device-to-device copies of this many "
                          "MiB (the calibration kernel for HBM byte counters)")
@@ -326,6 +422,10 @@ def main():
                                             if "fp64" in results else None),
                         "stages_ms_per_solve": {k: v["ms"] / args.steps
                                                 for k, v in mr["prof"].items() if v["count"]}}
+    if not args.no_stages and "fp64" in results:
+        # R of the last FP64 step is resident in dCtm; Cmm0 is the regenerated Cmm
+        out["stages"] = stages(tomo, n, m, dCmm0, dCtm0, dCtm)
+        print(f"[bench] stages: {json.dumps(out['stages'])}", file=sys.stderr, flush=True)
     if args.serve > 0:
         out["serving"] = []
         for sm in args.serve_modes.split(","):

@@ -7,13 +7,15 @@
 // flag and return at once).  The inverse L^{-1} (lower, zero above the diagonal) goes to a
 // separate kLeaf x kLeaf buffer and turns every panel triangular solve into an MFMA GEMM.
 //
-// Both halves are blocked by kB = 16 columns so a leaf needs ~6 workgroup barriers per block
-// instead of ~3 per column:
-//   factor: [wave 0] unblocked Cholesky of the 16x16 diagonal block (wave-synchronous LDS);
-//           [all] rows below solve against it by substitution (one lane per row);
-//           [all] 2x2-register-blocked rank-16 update of the trailing lower triangle.
+// One workgroup (4 waves) per leaf; the block is zero-padded to kLeaf x kLeaf in LDS and
+// processed by 16-column blocks:
+//   load:   batched global loads (16 in flight per lane) before the LDS stores;
+//   factor: [wave 0] 16x16 diagonal block in registers (readlane broadcasts, reciprocal of
+//           each pivot kept for the later divisions); [all] rows below by substitution (one
+//           lane per row); [all] rank-16 update of the trailing lower triangle on f64 MFMA;
 //   invert: block columns from last to first: [wave 0] D = inv(L_jj) by substitution;
-//           [all] T = inv(L)[below, below] * L[below, j];  [all] column j of inv(L) = -T * D.
+//           [all] T = inv(L)[below, below] * L[below, j] and column j of inv(L) = -T * D, both
+//           on f64 MFMA, 16-row tiles per wave.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -29,9 +31,31 @@ constexpr int kB = 16;         // inner block
 
 #define S_(r, c) S[(c) * LD + (r)]
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Value of v in lane l (l wave-uniform).
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// 16 x 16 block product on MFMA: acc += X[r0:r0+16, k0:k0+4*nk4] * Y[k0:k0+4*nk4, c0:c0+16] with
+// X element (r, k) at X[r + k*ldx] and Y element (k, c) at Y[yk*k + yc*c] (LDS).  Lane l
+// supplies X[r0 + l%16][k] and Y[k][c0 + l%16] for k = k0 + 4q + l/16; accumulator item t of
+// lane l is row l/16 + 4t, column l%16.
+__device__ __forceinline__ f64x4 mfma_block(const double* X, int ldx, int r0, const double* Y,
+                                             int yk, int yc, int c0, int k0, int nk4, f64x4 acc,
+                                             int lane) {
+    const int li = lane & 15, lk = lane >> 4;
+    for (int q = 0; q < nk4; ++q) {
+        const int k = k0 + 4 * q + lk;
+        const double a = X[(r0 + li) + k * ldx];
+        const double b = Y[yk * k + yc * (c0 + li)];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
 }
 
 template <typename TS>
@@ -40,41 +64,72 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
     __shared__ double S[kLeaf * LD];
     __shared__ double T[(kLeaf - kB) * kB];  // inverse: product of the trailing part
     __shared__ double D[kB * kB];            // inverse of one diagonal block
+    __shared__ double rdiag[kLeaf];          // 1 / L_jj
     __shared__ int fail;
     if (*(volatile int*)info != 0) return;  // an earlier leaf failed: R is not computed
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) fail = 0;
-    // a wave per column, lanes down the rows (no runtime division in the index math)
-    for (int c = wave; c < nb; c += NTL / 64)
-        for (int r = lane; r < nb; r += 64) S_(r, c) = (r >= c) ? (double)A[r + (long)c * lda] : 0.0;
+    // Load the lower triangle, zero-padded to kLeaf x kLeaf.  A wave owns columns
+    // wave, wave+4, ...; each batch issues 16 independent loads per lane before any LDS store
+    // (addresses clamped into the block, values outside the lower triangle replaced by 0).
+    {
+        constexpr int CB = 8;
+        const int rA = min(lane, nb - 1), rB = min(lane + 64, nb - 1);
+        for (int cb = wave; cb < kLeaf; cb += 4 * CB) {
+            double v0[CB], v1[CB];
+#pragma unroll
+            for (int q = 0; q < CB; ++q) {
+                const long cc = min(cb + 4 * q, nb - 1);
+                v0[q] = (double)A[rA + cc * lda];
+                v1[q] = (double)A[rB + cc * lda];
+            }
+#pragma unroll
+            for (int q = 0; q < CB; ++q) {
+                const int c = cb + 4 * q;
+                S_(lane, c) = (c < nb && lane < nb && lane >= c) ? v0[q] : 0.0;
+                S_(lane + 64, c) = (c < nb && lane + 64 < nb && lane + 64 >= c) ? v1[q] : 0.0;
+            }
+        }
+    }
     __syncthreads();
 
     // ------------------------------------------------------------------ factorization
     for (int kb = 0; kb < nb; kb += kB) {
         const int bs = min(kB, nb - kb);
         if (wave == 0) {
-            const int i = lane & 15, kq = lane >> 4;
-            for (int j = 0; j < bs; ++j) {
-                const double d = S_(kb + j, kb + j);
-                if (!(d > 0.0)) {  // same value in every lane
-                    if (lane == 0) {
-                        fail = 1;
-                        atomicCAS(info, 0, j0 + kb + j + 1);
+            // Diagonal block: lane i holds row i in registers; column j is broadcast with
+            // readlane (uniform lane index), so the block needs no LDS round trips.
+            const int i = lane;
+            double a[kB];
+#pragma unroll
+            for (int k = 0; k < kB; ++k) a[k] = (i < bs && k <= i) ? S_(kb + i, kb + k) : 0.0;
+            bool bad = false;
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                if (j < bs && !bad) {
+                    const double d = lane_value(a[j], j);
+                    if (!(d > 0.0)) {  // same value in every lane
+                        bad = true;
+                        if (lane == 0) {
+                            fail = 1;
+                            atomicCAS(info, 0, j0 + kb + j + 1);
+                        }
+                    } else {
+                        const double sd = sqrt(d), rsd = 1.0 / sd;
+                        a[j] = (i == j) ? sd : a[j] * rsd;
+                        if (lane == 0) rdiag[kb + j] = rsd;
+#pragma unroll
+                        for (int k = j + 1; k < kB; ++k) {
+                            const double lkj = lane_value(a[j], k);
+                            if (i >= k) a[k] -= a[j] * lkj;
+                        }
                     }
-                    break;
                 }
-                const double sd = sqrt(d);
-                wave_sync();
-                if (lane > j && lane < bs) S_(kb + lane, kb + j) /= sd;
-                if (lane == 0) S_(kb + j, kb + j) = sd;
-                wave_sync();
-                if (i > j && i < bs) {
-                    const double lij = S_(kb + i, kb + j);
-                    for (int k = j + 1 + kq; k <= i; k += 4)
-                        S_(kb + i, kb + k) -= lij * S_(kb + k, kb + j);
-                }
-                wave_sync();
             }
+            if (i < bs)
+#pragma unroll
+                for (int k = 0; k < kB; ++k)
+                    if (k <= i) S_(kb + i, kb + k) = a[k];
         }
         __syncthreads();
         if (fail) return;
@@ -91,7 +146,7 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
                     double v = x[c];
 #pragma unroll
                     for (int k = 0; k < c; ++k) v -= x[k] * S_(kb + c, kb + k);
-                    x[c] = v / S_(kb + c, kb + c);
+                    x[c] = v * rdiag[kb + c];
                 }
             }
 #pragma unroll
@@ -99,31 +154,26 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
                 if (c < bs) S_(r, kb + c) = x[c];
         }
         __syncthreads();
-        // trailing update of the lower triangle, 2x2 register blocks
+        // trailing update of the lower triangle on MFMA: 16 x 16 tiles (ti >= tj) of the
+        // trailing part, K = 16 (bs is 16 whenever rows remain below the block)
         if (rest > 0) {
-            const int h = (rest + 1) / 2;
-            const int nblk = h * (h + 1) / 2;
-            const int base = kb + bs;
-            for (int blk = tid; blk < nblk; blk += NTL) {
-                int bi = (int)((sqrtf(8.0f * blk + 1.0f) - 1.0f) * 0.5f);
-                while ((bi + 1) * (bi + 2) / 2 <= blk) ++bi;
-                while (bi * (bi + 1) / 2 > blk) --bi;
-                const int bj = blk - bi * (bi + 1) / 2;
-                const int r0 = base + 2 * bi, c0 = base + 2 * bj;
-                const bool r1ok = r0 + 1 < nb, c1ok = c0 + 1 < nb;
-                double a00 = 0, a01 = 0, a10 = 0, a11 = 0;
-                for (int k = 0; k < bs; ++k) {
-                    const double p0 = S_(r0, kb + k), p1 = r1ok ? S_(r0 + 1, kb + k) : 0.0;
-                    const double q0 = S_(c0, kb + k), q1 = c1ok ? S_(c0 + 1, kb + k) : 0.0;
-                    a00 += p0 * q0;
-                    a01 += p0 * q1;
-                    a10 += p1 * q0;
-                    a11 += p1 * q1;
+            const int base = kb + kB;
+            const int ht = (rest + 15) / 16;
+            const int ntile = ht * (ht + 1) / 2;
+            for (int t = wave; t < ntile; t += NTL / 64) {
+                int ti = 0, acc0 = 0;
+                while (acc0 + ti + 1 <= t) acc0 += ++ti;
+                const int tj = t - acc0;
+                f64x4 acc = {0, 0, 0, 0};
+                // P[r][k] = S_(r, kb + k); B operand is P^T: element (k, c) = S_(c, kb + k)
+                acc = mfma_block(&S_(0, kb), LD, base + 16 * ti, &S_(0, kb), LD, 1, base + 16 * tj,
+                                 0, kB / 4, acc, lane);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = base + 16 * ti + (lane >> 4) + 4 * e;
+                    const int c = base + 16 * tj + (lane & 15);
+                    if (r < nb && c < nb && r >= c) S_(r, c) -= acc[e];
                 }
-                S_(r0, c0) -= a00;
-                if (c1ok && r0 >= c0 + 1) S_(r0, c0 + 1) -= a01;
-                if (r1ok) S_(r0 + 1, c0) -= a10;
-                if (r1ok && c1ok) S_(r0 + 1, c0 + 1) -= a11;
             }
         }
         __syncthreads();
@@ -137,6 +187,7 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
     for (int jb = nblocks - 1; jb >= 0; --jb) {
         const int c0 = jb * kB, bs = min(kB, nb - c0);
         const int below = nb - c0 - bs;
+        const int kbeg = c0 + bs;
         // D = inv(L_jj): lane c computes column c by forward substitution
         if (wave == 0 && lane < bs) {
             const int c = lane;
@@ -149,32 +200,39 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
 #pragma unroll
                     for (int k = 0; k < r; ++k)
                         if (k >= c) v -= S_(c0 + r, c0 + k) * x[k];
-                    v /= S_(c0 + r, c0 + r);
+                    v *= rdiag[c0 + r];
                 }
                 x[r] = v;
             }
 #pragma unroll
             for (int r = 0; r < kB; ++r) D[r * kB + c] = x[r];  // D[r][c]
         }
-        // T = inv(L)[below, below] * L[below, c0:c0+bs]   (inv(L) trailing part already in S)
-        // thread -> (row rr = tid/16 + 16 i, column c = tid%16): no runtime division
-        const int tc = tid & (kB - 1), tr = tid / kB;
-        if (tc < bs)
-            for (int rr = tr; rr < below; rr += NTL / kB) {
-                const int r = c0 + bs + rr;
-                double v = 0.0;
-                for (int k = c0 + bs; k <= r; ++k) v += S_(r, k) * S_(k, c0 + tc);
-                T[tc * (kLeaf - kB) + rr] = v;
-            }
+        // T = inv(L)[below, below] * L[below, c0:c0+16] on MFMA, a 16-row tile per wave turn;
+        // inv(L) of the trailing part is lower triangular (zero above, zero padding below nb),
+        // so row tile i needs k < 16 (i + 1) only
+        const int nti = (below + 15) / 16;
+        for (int i = wave; i < nti; i += NTL / 64) {
+            f64x4 acc = {0, 0, 0, 0};
+            acc = mfma_block(&S_(0, kbeg), LD, 16 * i + kbeg, &S_(kbeg, c0), 1, LD, 0, 0,
+                             4 * (i + 1), acc, lane);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                T[(lane & 15) * (kLeaf - kB) + 16 * i + (lane >> 4) + 4 * e] = acc[e];
+        }
         __syncthreads();
         // column block jb of inv(L): rows below = -T * D; diagonal block = D
-        if (tc < bs) {
-            for (int rr = tr; rr < below; rr += NTL / kB) {
-                double v = 0.0;
-                for (int k = tc; k < bs; ++k) v += T[k * (kLeaf - kB) + rr] * D[k * kB + tc];
-                S_(c0 + bs + rr, c0 + tc) = -v;
+        for (int i = wave; i < nti; i += NTL / 64) {
+            f64x4 acc = {0, 0, 0, 0};
+            acc = mfma_block(T, kLeaf - kB, 16 * i, D, kB, 1, 0, 0, kB / 4, acc, lane);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int rr = 16 * i + (lane >> 4) + 4 * e;
+                if (rr < below) S_(kbeg + rr, c0 + (lane & 15)) = -acc[e];
             }
-            if (tr < bs) S_(c0 + tr, c0 + tc) = tr >= tc ? D[tr * kB + tc] : 0.0;
+        }
+        {
+            const int tc = tid & (kB - 1), tr = tid / kB;
+            if (tc < bs && tr < bs) S_(c0 + tr, c0 + tc) = tr >= tc ? D[tr * kB + tc] : 0.0;
         }
         __syncthreads();
     }
