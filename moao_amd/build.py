@@ -1,6 +1,7 @@
 """Builds libmoao_hip.so (gfx950) in-tree: moao_amd/_lib/libmoao_hip.so.
 
-    python -m moao_amd.build            # compile every source, link the shared library
+    python -m moao_amd.build            # compile every source, link the shared library and
+                                        # the command-line driver moao_amd/_lib/moao
 
 hipcc compiles each source of moao_amd/csrc/ to an object (in parallel), then links them.
 Every HIP compile carries -Xclang -target-feature -Xclang -packed-fp32-ops so the compiler
@@ -62,8 +63,17 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the command-line driver (C++ over the C-ABI only), next to the library
+    cli = os.path.join(OUT_DIR, "moao")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(HERE, "cli", "moao_main.cpp"), "-o", cli + ".tmp", f"-L{OUT_DIR}",
+           "-lmoao_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(cli + ".tmp", cli)
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources")
+        print(f"built {LIB} from {len(srcs)} sources and {cli}")
     return LIB
 
 
