@@ -2,7 +2,7 @@
 //
 // Workgroup: 256 lanes = 4 waves in a 2x2 arrangement over a 128x128 C tile; each wave owns a
 // 64x64 sub-tile = 4x4 MFMA blocks of 16x16 held in accumulator registers.  K advances in
-// slices of 16 through double-buffered LDS (k-major, m/n contiguous, padded rows), with the
+// slices of 8 through double-buffered LDS (k-major, m/n contiguous, padded rows), with the
 // next slice's global loads issued before the current slice's MFMAs so one barrier per slice
 // suffices.  MFMA operand layout (checked on the device): lane l supplies A[l%16][k0+l/16] and
 // B[k0+l/16][l%16]; accumulator item t of lane l is C[r][l%16] with r = 4*(l/16)+t for f32
@@ -18,7 +18,7 @@
 namespace moao {
 namespace {
 
-constexpr int BM = 128, BN = 128, BK = 16;
+constexpr int BM = 128, BN = 128, BK = 8;
 constexpr int PAD = 2;
 constexpr int NT = 256;
 
@@ -79,11 +79,12 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     // the per-lane part is a row offset computed once per tile: no per-load address VALU.
     // Raw loads go to registers with a validity mask applied at the LDS store (after the
     // slice's MFMAs), so no wait sits between the loads.
-    T ra[8], rb[8];
+    constexpr int RV = BK / 2, CP = NT / BK;  // values per thread, columns per pass
+    T ra[RV], rb[RV];
     unsigned ma = 0, mb = 0;
     const int lrow = tid & 127;
     const int lk = __builtin_amdgcn_readfirstlane(tid >> 7);  // 0 for waves 0-1, 1 for 2-3
-    const int bk = tid & 15, bn = tid >> 4;                   // B (K x N) pattern
+    const int bk = tid % BK, bn = tid / BK;                   // B (K x N) pattern
     const long lda = s.lda, ldb = g.ldb;
     const int arow = m0 + lrow;
     const unsigned aoff = (unsigned)min(arow, Mc) * (unsigned)sizeof(T);  // bytes
@@ -91,9 +92,9 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     const T* __restrict__ A = s.A;
     const T* __restrict__ Bm = g.B;
     // TB=0 / crossing-slice pattern: per-lane column offsets, computed once per tile
-    long bcol[8];
+    long bcol[RV];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) bcol[r] = (long)min(n0 + bn + 16 * r, Nc) * ldb;
+    for (int r = 0; r < RV; ++r) bcol[r] = (long)min(n0 + bn + CP * r, Nc) * ldb;
     const unsigned bcoloff = (unsigned)min(n0 + lrow, Nc) * (unsigned)sizeof(T);  // bytes
     const bool bcol_ok = n0 + lrow < N;
 
@@ -101,7 +102,7 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     auto load = [&](int k0) {
         ma = mb = 0;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
+        for (int r = 0; r < RV; ++r) {
             const int k = k0 + lk + 2 * r;
             const T* col = A + (long)min(k, Kc) * lda;  // wave-uniform
             ra[r] = *(const T*)((const char*)col + aoff);
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
         bpat = pat == 1 ? 1 : 0;
         if (pat == 1) {
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
+            for (int r = 0; r < RV; ++r) {
                 const int k = k0 + lk + 2 * r;
                 const T* col = Bm + (long)min(k, Kc) * ldb;  // wave-uniform
                 rb[r] = *(const T*)((const char*)col + bcoloff);
@@ -125,31 +126,31 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
         } else if (pat == 0) {
             const int k = k0 + bk, kc = min(k, Kc);
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
+            for (int r = 0; r < RV; ++r) {
                 rb[r] = Bm[bcol[r] + kc];
-                mb |= (unsigned)(n0 + bn + 16 * r < N && k < K) << r;
+                mb |= (unsigned)(n0 + bn + CP * r < N && k < K) << r;
             }
         } else {
             const int k = k0 + bk, kc = min(k, Kc);
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const int cc = min(n0 + bn + 16 * r, Nc);
+            for (int r = 0; r < RV; ++r) {
+                const int cc = min(n0 + bn + CP * r, Nc);
                 rb[r] = kc >= cc ? Bm[bcol[r] + kc] : Bm[cc + (long)kc * ldb];
-                mb |= (unsigned)(n0 + bn + 16 * r < N && k < K) << r;
+                mb |= (unsigned)(n0 + bn + CP * r < N && k < K) << r;
             }
         }
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) As[buf][lk + 2 * r][lrow] = (ma >> r & 1) ? ra[r] : T(0);
+        for (int r = 0; r < RV; ++r) As[buf][lk + 2 * r][lrow] = (ma >> r & 1) ? ra[r] : T(0);
         if (bpat) {
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
+            for (int r = 0; r < RV; ++r)
                 Bs[buf][lk + 2 * r][lrow] = (mb >> r & 1) ? rb[r] : T(0);
         } else {
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
-                Bs[buf][bk][bn + 16 * r] = (mb >> r & 1) ? rb[r] : T(0);
+            for (int r = 0; r < RV; ++r)
+                Bs[buf][bk][bn + CP * r] = (mb >> r & 1) ? rb[r] : T(0);
         }
     };
 
