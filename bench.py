@@ -427,6 +427,14 @@ def main():
     if not args.no_stages and "fp64" in results:
         # R of the last FP64 step is resident in dCtm; Cmm0 is the regenerated Cmm
         out["stages"] = stages(tomo, n, m, dCmm0, dCtm0, dCtm)
+        L = tomo.dims()["nLayer"]
+        elems = n * (n + 1) / 2 + m * n
+        mfl = (30 + 115 * L) * n * (n + 1) / 2 + (30 + 103 * L) * m * n  # chameleon/flops.h
+        out["stages"]["matcov"] = {
+            "elements": elems, "ms": mc["ms"], "elements_per_s": elems / (mc["ms"] * 1e-3),
+            "GBps_written": 8 * elems / (mc["ms"] * 1e-3) / 1e9,
+            "model_flops": mfl, "model_tflops": mfl / (mc["ms"] * 1e-3) / 1e12,
+            "bound": "fp64 valu (transcendentals)", "peak_fp64_vector_tflops": 78.6}
         print(f"[bench] stages: {json.dumps(out['stages'])}", file=sys.stderr, flush=True)
     if args.serve > 0:
         out["serving"] = []

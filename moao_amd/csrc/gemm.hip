@@ -6,7 +6,9 @@
 // next slice's global loads issued before the current slice's MFMAs so one barrier per slice
 // suffices.  MFMA operand layout (checked on the device): lane l supplies A[l%16][k0+l/16] and
 // B[k0+l/16][l%16]; accumulator item t of lane l is C[r][l%16] with r = 4*(l/16)+t for f32
-// and r = l/16 + 4t for f64.
+// and r = l/16 + 4t for f64.  A launch with fewer than two 128-row tiles per CU (panel solves
+// of the leaves, narrow updates) uses 64-row tiles instead (each wave 32x64, up to 4
+// workgroups per CU), which doubles the workgroups that fill the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,18 +50,21 @@ struct Mfma<float> {
 // TB: 0 = B is K x N, 1 = B is N x K (op = transpose), 2 = B symmetric K x K, lower stored.
 // Loads are branch-free: addresses are clamped into the matrix and out-of-range values zeroed
 // by a select, so the compiler issues every load of a slice back to back.
-template <typename T, int TB>
-__global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
+// WMB: 16-row MFMA blocks per wave along M (4: 128-row tile; 2: 64-row tile for launches
+// too small to fill the device with 128-row tiles).
+template <typename T, int TB, int WMB>
+__global__ void __launch_bounds__(NT, WMB == 4 ? 2 : 4) gemm_kernel(GemmArgs<T> g) {
     using acc_t = typename Mfma<T>::acc_t;
-    __shared__ T As[2][BK][BM + PAD];
+    constexpr int TM = 32 * WMB;  // rows of the C tile
+    __shared__ T As[2][BK][TM + PAD];
     __shared__ T Bs[2][BK][BN + PAD];
 
     const GemmSeg<T> s = g.seg[blockIdx.z];
-    const int m0 = blockIdx.x * BM;
+    const int m0 = blockIdx.x * TM;
     const int n0 = blockIdx.y * BN;
     if (m0 >= s.M) return;
     const bool tri = g.tri && blockIdx.z == 0;
-    if (tri && m0 + BM - 1 + g.tri_off < n0) return;
+    if (tri && m0 + TM - 1 + g.tri_off < n0) return;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -68,9 +73,9 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     const int K = g.K, N = g.N, M = s.M;
     const int Mc = M - 1, Nc = N - 1, Kc = K > 0 ? K - 1 : 0;
 
-    acc_t acc[4][4];
+    acc_t acc[WMB][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WMB; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
@@ -79,14 +84,17 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     // the per-lane part is a row offset computed once per tile: no per-load address VALU.
     // Raw loads go to registers with a validity mask applied at the LDS store (after the
     // slice's MFMAs), so no wait sits between the loads.
-    constexpr int RV = BK / 2, CP = NT / BK;  // values per thread, columns per pass
-    T ra[RV], rb[RV];
+    constexpr int RV = BK / 2, CP = NT / BK;  // B values per thread, columns per pass
+    constexpr int KA = NT / TM, RA = BK / KA;  // A: k columns per pass, values per thread
+    T ra[RA], rb[RV];
     unsigned ma = 0, mb = 0;
     const int lrow = tid & 127;
     const int lk = __builtin_amdgcn_readfirstlane(tid >> 7);  // 0 for waves 0-1, 1 for 2-3
+    const int arl = tid % TM;
+    const int akl = __builtin_amdgcn_readfirstlane(tid / TM);
     const int bk = tid % BK, bn = tid / BK;                   // B (K x N) pattern
     const long lda = s.lda, ldb = g.ldb;
-    const int arow = m0 + lrow;
+    const int arow = m0 + arl;
     const unsigned aoff = (unsigned)min(arow, Mc) * (unsigned)sizeof(T);  // bytes
     const bool arow_ok = arow < M;
     const T* __restrict__ A = s.A;
@@ -102,8 +110,8 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     auto load = [&](int k0) {
         ma = mb = 0;
 #pragma unroll
-        for (int r = 0; r < RV; ++r) {
-            const int k = k0 + lk + 2 * r;
+        for (int r = 0; r < RA; ++r) {
+            const int k = k0 + akl + KA * r;
             const T* col = A + (long)min(k, Kc) * lda;  // wave-uniform
             ra[r] = *(const T*)((const char*)col + aoff);
             ma |= (unsigned)(arow_ok && k < K) << r;
@@ -142,7 +150,7 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < RV; ++r) As[buf][lk + 2 * r][lrow] = (ma >> r & 1) ? ra[r] : T(0);
+        for (int r = 0; r < RA; ++r) As[buf][akl + KA * r][arl] = (ma >> r & 1) ? ra[r] : T(0);
         if (bpat) {
 #pragma unroll
             for (int r = 0; r < RV; ++r)
@@ -161,7 +169,7 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     }
     __syncthreads();
     int cur = 0;
-    const int ar = wm * 64 + (lane & 15);
+    const int ar = wm * 16 * WMB + (lane & 15);
     const int bc = wn * 64 + (lane & 15);
     const int kq = lane >> 4;
     for (int kt = 0; kt < nk; ++kt) {
@@ -169,13 +177,13 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
 #pragma unroll
         for (int kk = 0; kk < BK / 4; ++kk) {
             const int k = kk * 4 + kq;
-            T a[4], b[4];
+            T a[WMB], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[cur][k][ar + 16 * i];
+            for (int i = 0; i < WMB; ++i) a[i] = As[cur][k][ar + 16 * i];
 #pragma unroll
             for (int j = 0; j < 4; ++j) b[j] = Bs[cur][k][bc + 16 * j];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < WMB; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mac(a[i], b[j], acc[i][j]);
         }
@@ -187,14 +195,14 @@ __global__ void __launch_bounds__(NT, 2) gemm_kernel(GemmArgs<T> g) {
     // Epilogue: C = alpha*acc + beta*C (C not read when beta == 0).
     const T alpha = g.alpha, beta = g.beta;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < WMB; ++i) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int col = n0 + wn * 64 + j * 16 + (lane & 15);
             if (col >= N) continue;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const int row = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, t);
+                const int row = m0 + wm * 16 * WMB + i * 16 + Mfma<T>::row(lane, t);
                 if (row >= M) continue;
                 if (tri && row + g.tri_off < col) continue;
                 T* c = s.C + row + (long)col * s.ldc;
@@ -234,12 +242,22 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
         }
     }
     ProfScope ps(sizeof(T) == 8 ? kProfGemmF64 : kProfGemmF32, work, stream);
-    if (g.transb == 0)
-        hipLaunchKernelGGL((gemm_kernel<T, 0>), grid, dim3(NT), 0, stream, g);
+    // 128-row tiles when they fill the device (>= 2 per CU), else 64-row tiles
+    const bool small = (long)grid.x * grid.y * grid.z < 2L * 256;
+    if (small) {
+        grid.x = (maxM + 63) / 64;
+        if (g.transb == 0)
+            hipLaunchKernelGGL((gemm_kernel<T, 0, 2>), grid, dim3(NT), 0, stream, g);
+        else if (g.transb == 1)
+            hipLaunchKernelGGL((gemm_kernel<T, 1, 2>), grid, dim3(NT), 0, stream, g);
+        else
+            hipLaunchKernelGGL((gemm_kernel<T, 2, 2>), grid, dim3(NT), 0, stream, g);
+    } else if (g.transb == 0)
+        hipLaunchKernelGGL((gemm_kernel<T, 0, 4>), grid, dim3(NT), 0, stream, g);
     else if (g.transb == 1)
-        hipLaunchKernelGGL((gemm_kernel<T, 1>), grid, dim3(NT), 0, stream, g);
+        hipLaunchKernelGGL((gemm_kernel<T, 1, 4>), grid, dim3(NT), 0, stream, g);
     else
-        hipLaunchKernelGGL((gemm_kernel<T, 2>), grid, dim3(NT), 0, stream, g);
+        hipLaunchKernelGGL((gemm_kernel<T, 2, 4>), grid, dim3(NT), 0, stream, g);
     MOAO_HIP_TRY(hipGetLastError());
     return MOAO_OK;
 }

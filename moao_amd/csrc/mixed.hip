@@ -14,6 +14,8 @@
 //      stop when ||E||_F <= tol ||Ctm||_F, else D = E C0^{-1} (FP32 solves) corrected by
 //      Woodbury, R += D.
 // Without a low-rank description (p = 0) the same loop runs on Cmm itself.
+// With the environment variable MOAO_DEBUG set, the norms of the intermediate terms and the
+// residual of every refinement step are printed to stderr (synchronising after each).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
