@@ -10,9 +10,10 @@
 // One workgroup (4 waves) per leaf; the block is zero-padded to kLeaf x kLeaf in LDS and
 // processed by 16-column blocks:
 //   load:   batched global loads (16 in flight per lane) before the LDS stores;
-//   factor: [wave 0] 16x16 diagonal block in registers (readlane broadcasts, reciprocal of
-//           each pivot kept for the later divisions); [all] rows below by substitution (one
-//           lane per row); [all] rank-16 update of the trailing lower triangle on f64 MFMA;
+//   factor: [wave 0] 16x16 diagonal block in registers (readlane broadcasts, 1/sqrt of each
+//           pivot by a refined hardware estimate, kept for the later divisions); [all] rows
+//           below by right-looking substitution (one lane per row); [all] rank-16 update of
+//           the trailing lower triangle on f64 MFMA;
 //   invert: block columns from last to first: [wave 0] D = inv(L_jj) by substitution;
 //           [all] T = inv(L)[below, below] * L[below, j] and column j of inv(L) = -T * D, both
 //           on f64 MFMA, 16-row tiles per wave.
@@ -39,6 +40,15 @@ __device__ __forceinline__ double lane_value(double v, int l) {
     const int lo = __builtin_amdgcn_readlane((int)b, l);
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// 1/sqrt(d) for d > 0: hardware estimate refined by two Newton steps (no division, no
+// correctly-rounded sqrt sequence on the pivot's critical path).
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    double r = __builtin_amdgcn_rsq(d);
+    r = r * (1.5 - 0.5 * d * r * r);
+    r = r * (1.5 - 0.5 * d * r * r);
+    return r;
 }
 
 // 16 x 16 block product on MFMA: acc += X[r0:r0+16, k0:k0+4*nk4] * Y[k0:k0+4*nk4, c0:c0+16] with
@@ -115,7 +125,7 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
                             atomicCAS(info, 0, j0 + kb + j + 1);
                         }
                     } else {
-                        const double sd = sqrt(d), rsd = 1.0 / sd;
+                        const double rsd = rsqrt_nr(d), sd = d * rsd;
                         a[j] = (i == j) ? sd : a[j] * rsd;
                         if (lane == 0) rdiag[kb + j] = rsd;
 #pragma unroll
@@ -135,23 +145,22 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
         if (fail) return;
         // rows below the diagonal block: x * L_bb^T = a  (forward substitution per row)
         const int rest = nb - kb - bs;
+        // (bs is 16 whenever rows remain below the block).  Right-looking: once x[c] is known
+        // every later x[c2] takes its term at once, so each step is one multiply followed by
+        // independent FMAs instead of a dot-product chain.
         if (tid < rest) {
-            const int r = kb + bs + tid;
+            const int r = kb + kB + tid;
             double x[kB];
 #pragma unroll
-            for (int c = 0; c < kB; ++c) x[c] = c < bs ? S_(r, kb + c) : 0.0;
+            for (int c = 0; c < kB; ++c) x[c] = S_(r, kb + c);
 #pragma unroll
             for (int c = 0; c < kB; ++c) {
-                if (c < bs) {
-                    double v = x[c];
+                x[c] *= rdiag[kb + c];
 #pragma unroll
-                    for (int k = 0; k < c; ++k) v -= x[k] * S_(kb + c, kb + k);
-                    x[c] = v * rdiag[kb + c];
-                }
+                for (int c2 = c + 1; c2 < kB; ++c2) x[c2] -= x[c] * S_(kb + c2, kb + c);
             }
 #pragma unroll
-            for (int c = 0; c < kB; ++c)
-                if (c < bs) S_(r, kb + c) = x[c];
+            for (int c = 0; c < kB; ++c) S_(r, kb + c) = x[c];
         }
         __syncthreads();
         // trailing update of the lower triangle on MFMA: 16 x 16 tiles (ti >= tj) of the
@@ -189,20 +198,17 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
         const int below = nb - c0 - bs;
         const int kbeg = c0 + bs;
         // D = inv(L_jj): lane c computes column c by forward substitution
-        if (wave == 0 && lane < bs) {
+        // (right-looking from e_c: rows above c stay 0; rows at or past bs are zero padding)
+        if (wave == 0 && lane < kB) {
             const int c = lane;
             double x[kB];
 #pragma unroll
-            for (int r = 0; r < kB; ++r) {
-                double v = 0.0;
-                if (r >= c && r < bs) {
-                    v = (r == c) ? 1.0 : 0.0;
+            for (int r = 0; r < kB; ++r) x[r] = (r == c) ? 1.0 : 0.0;
 #pragma unroll
-                    for (int k = 0; k < r; ++k)
-                        if (k >= c) v -= S_(c0 + r, c0 + k) * x[k];
-                    v *= rdiag[c0 + r];
-                }
-                x[r] = v;
+            for (int r = 0; r < kB; ++r) {
+                x[r] = r < bs ? x[r] * rdiag[c0 + r] : 0.0;
+#pragma unroll
+                for (int r2 = r + 1; r2 < kB; ++r2) x[r2] -= S_(c0 + r2, c0 + r) * x[r];
             }
 #pragma unroll
             for (int r = 0; r < kB; ++r) D[r * kB + c] = x[r];  // D[r][c]
