@@ -406,7 +406,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
                      "traffic": hbm_traffic(args.workload),
-                     "kernel": "gemm_kernel<double> (all launches of the timed solves)",
+                     "kernel": "FP64 GEMM engine: gemm_dl_kernel + gemm_kernel (all launches of the timed solves)",
                      "launches": g["count"], "avg_launch_ms": g["ms"] / max(1, g["count"])},
         "stages_ms_per_solve": {k: v["ms"] / args.steps for k, v in r["prof"].items()
                                 if v["count"]},
