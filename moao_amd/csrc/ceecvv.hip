@@ -45,6 +45,19 @@ int cee_cvv_dev(long n, long m, long nact, const double* Cmm, long ldCmm, double
         return MOAO_ERR_ARG;
     }
     const int N = (int)n, M = (int)m, A = (int)nact;
+    // Stream-ordered scratch for the symmetric operands' diagonal blocks (steps 2 and 7), so
+    // calls on different streams never share it; without it those steps use the
+    // register-staged kernel.
+    struct Scratch {
+        double* p = nullptr;
+        hipStream_t s;
+        ~Scratch() { if (p) (void)hipFreeAsync(p, s); }
+    } sd{nullptr, s};
+    if (hipMallocAsync(reinterpret_cast<void**>(&sd.p),
+                       sizeof(double) * gemm_symdiag_elems(std::max(n, m)), s) != hipSuccess) {
+        sd.p = nullptr;
+        (void)hipGetLastError();
+    }
     // 1: Cpp(lower) -= Cpm R^T + R Cpm^T
     auto g = gemm_args<double>(M, M, N, -1.0, Cpm, ldCpm, R, ldR, 1, 1.0, Cpp, ldCpp);
     g.tri = 1;
@@ -54,6 +67,10 @@ int cee_cvv_dev(long n, long m, long nact, const double* Cmm, long ldCmm, double
     if (int rc = gemm_f64(g, s)) return rc;
     // 2: Cpm = R sym(Cmm)
     g = gemm_args<double>(M, N, N, 1.0, R, ldR, Cmm, ldCmm, 2, 0.0, Cpm, ldCpm);
+    if (sd.p) {
+        if (int rc = gemm_symdiag(Cmm, ldCmm, N, sd.p, s)) return rc;
+        g.symdiag = sd.p;
+    }
     if (int rc = gemm_f64(g, s)) return rc;
     // 3-5: Cee = lower(Cpp) mirrored
     {
@@ -68,6 +85,10 @@ int cee_cvv_dev(long n, long m, long nact, const double* Cmm, long ldCmm, double
     if (A == 0) return MOAO_OK;
     // 7: Tmp = Dx sym(lower(Cee))
     g = gemm_args<double>(A, M, M, 1.0, Dx, ldDx, Cee, ldCee, 2, 0.0, Tmp, ldTmp);
+    if (sd.p) {
+        if (int rc = gemm_symdiag(Cee, ldCee, M, sd.p, s)) return rc;
+        g.symdiag = sd.p;
+    }
     if (int rc = gemm_f64(g, s)) return rc;
     // 8: Cvv = Tmp Dx^T
     g = gemm_args<double>(A, A, M, 1.0, Tmp, ldTmp, Dx, ldDx, 1, 0.0, Cvv, ldCvv);

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 #include "gemm.hpp"
@@ -230,10 +231,14 @@ __global__ void __launch_bounds__(NT, WMB == 4 ? 2 : 4) gemm_kernel(GemmArgs<T> 
 // matrix.  op(B) = B (K x N) tiles are
 // BK-row slabs of 128 columns: one descriptor per slab whose range ends at the slab's last
 // element, 4 (FP64) or 8 (FP32) columns per 256-byte load, column groups padded in LDS.
+// The LDS pointer passes through an empty asm so the compiler cannot trace it to one LDS array:
+// with that provenance it would put a vmcnt(0) before every later read of the same array and
+// serialize the pipeline.  Ordering is explicit instead: wait_vm + bar before each slice's reads.
 template <typename T>
 __device__ __forceinline__ void dl_load(__amdgpu_buffer_rsrc_t r, T* lds, int voff) {
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(
-        r, (__attribute__((address_space(3))) void*)lds, 4, voff, 0, 0, 0);
+    __attribute__((address_space(3))) void* l = (__attribute__((address_space(3))) void*)lds;
+    asm("" : "+s"(l));  // the 32-bit LDS address, so no generic-to-LDS conversion follows
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, l, 4, voff, 0, 0, 0);
 }
 template <int VM>
 __device__ __forceinline__ void wait_vm() {
@@ -246,19 +251,22 @@ __device__ __forceinline__ void bar() {
     asm volatile("" ::: "memory");
 }
 
-template <typename T, int TB, int S>
-__global__ void __launch_bounds__(NT, 2) gemm_dl_kernel(GemmArgs<T> g) {
+template <typename T, int TB, int S, int TM>
+__global__ void __launch_bounds__(NT, TM == 128 ? 2 : 4) gemm_dl_kernel(GemmArgs<T> g) {
     using acc_t = typename Mfma<T>::acc_t;
-    constexpr int TM = 128, LDC = TM + PAD;
-    constexpr int QI = TM * (int)sizeof(T) / 256;     // 256-byte pieces per tile column
-    constexpr int IPW = 2 * BK * QI / 4;               // load instructions per wave per stage
+    constexpr int WMB = TM / 32;                       // 16-row MFMA blocks per wave along M
+    constexpr int LDA = TM + PAD, LDB = BN + PAD;
+    constexpr int QA = TM * (int)sizeof(T) / 256;      // 256-byte pieces per A tile column
+    constexpr int QB = BN * (int)sizeof(T) / 256;      // ... per B^T tile column
     constexpr int CPI = 256 / (BK * (int)sizeof(T));   // K x N: columns per 256-byte load
     constexpr int LPC = 64 / CPI;                      // K x N: lanes per column
     constexpr int GS = CPI * BK + 16 / (int)sizeof(T);  // K x N: LDS elements per column group
     constexpr int NG = BN / CPI;                       // K x N: column groups per tile
-    static_assert(NG == 2 * IPW, "two waves cover the K x N slab");
-    constexpr int BSZ = TB == 1 ? BK * LDC : NG * GS;  // B elements per stage
-    __shared__ T As[S][BK][LDC];
+    constexpr int IPWA = BK / 2 * QA;                  // loads per A wave per slice
+    constexpr int IPWB = TB == 0 ? NG / 2 : BK / 2 * QB;  // loads per B wave per slice
+    static_assert(TB != 2 || BK / 2 * QB == NG / 2, "both B patterns issue the same loads");
+    constexpr int BSZ = TB == 1 ? BK * LDB : (TB == 0 ? NG * GS : std::max(BK * LDB, NG * GS));
+    __shared__ T As[S][BK][LDA];
     __shared__ T Bs[S][BSZ];
 
     const GemmSeg<T> s = g.seg[blockIdx.z];
@@ -274,49 +282,64 @@ __global__ void __launch_bounds__(NT, 2) gemm_dl_kernel(GemmArgs<T> g) {
     const int wm = wave & 1, wn = wave >> 1;
     const int K = g.K, N = g.N, M = s.M;
 
-    acc_t acc[4][4];
+    acc_t acc[WMB][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WMB; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    // waves 0-1 load A (4 columns each per stage), waves 2-3 load B (half a stage each)
+    // waves 0-1 load A (half the slice's columns each), waves 2-3 load B (half a slice each).
+    // B slices come as column panels (op(B) = B^T, and symmetric B above the tile's diagonal
+    // block: column k of the lower triangle, rows n0..) or as row slabs (op(B) = B, and
+    // symmetric B below the diagonal block: rows k of columns n0..); symmetric slices that
+    // cross the diagonal block read its full symmetric copy in g.symdiag.
     const bool isA = wave < 2;
-    const bool colpanel = isA || TB == 1;
-    const T* base = isA ? s.A + m0 : (TB == 1 ? g.B + n0 : g.B + (long)n0 * g.ldb);
+    const T* base = isA ? s.A + m0 : (TB == 0 ? g.B + (long)n0 * g.ldb : g.B + n0);
     const long ld = isA ? s.lda : g.ldb;
+    const T* diagblk = TB == 2 ? g.symdiag + (long)(n0 / BN) * BN * BN : nullptr;
     const int nv = isA ? min(M - m0, TM) : min(N - n0, BN);  // valid rows (cols for K x N)
     const int half = wave & 1;
     // K x N: lane -> (column lane/LPC of the group, dword lane%LPC of that column's slab)
     const int vkn = (lane / LPC) * (int)(ld * (long)sizeof(T)) + (lane % LPC) * 4;
     const int gstride = CPI * (int)(ld * (long)sizeof(T));  // bytes between column groups
 
+    // column panels: one descriptor per column (column k of src sits at src + (k-kofs)*lds),
+    // Q pieces of 256 bytes each
+    auto colpanel = [&](auto qc, T* dst, int ldl, const T* src, long lds, int kofs, int k0) {
+        constexpr int Q = decltype(qc)::value;
+#pragma unroll
+        for (int c = 0; c < BK / 2; ++c) {
+            const int kk = half * (BK / 2) + c;
+            const int kcl = min(k0 + kk, K - 1);  // clamp past K (masked at use)
+            const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(src + (long)(kcl - kofs) * lds), 0, nv * (int)sizeof(T), 0x00020000);
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                dl_load(r, (T*)((char*)(dst + kk * ldl) + q * 256), lane * 4 + q * 256);
+        }
+    };
     const int nk = (K + BK - 1) / BK;
     auto issue = [&](int kt) {
         const int k0 = kt * BK;
-        if (colpanel) {
-            T* dst = isA ? &As[kt % S][0][0] : &Bs[kt % S][0];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int kk = half * 4 + c;
-                const int kcl = min(k0 + kk, K - 1);  // clamp past K (masked at use)
-                const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-                    (void*)(base + (long)kcl * ld), 0, nv * (int)sizeof(T), 0x00020000);
-#pragma unroll
-                for (int q = 0; q < QI; ++q)
-                    dl_load(r, (T*)((char*)(dst + kk * LDC) + q * 256), lane * 4 + q * 256);
-            }
+        const bool below = TB == 2 && k0 >= n0 + BN;
+        const bool cross = TB == 2 && !below && k0 + BK > n0;
+        if (isA) {
+            colpanel(std::integral_constant<int, QA>{}, &As[kt % S][0][0], LDA, base, ld, 0, k0);
+        } else if (TB == 1 || (TB == 2 && !below)) {
+            colpanel(std::integral_constant<int, QB>{}, &Bs[kt % S][0], LDB,
+                     cross ? diagblk : base, cross ? (long)BN : ld, cross ? n0 : 0, k0);
         } else {
             // slab rows k0..k0+BK-1 of columns n0..n0+nv-1; the range ends at the last valid
             // element so nothing past the matrix is read
+            const T* slab = TB == 0 ? base : g.B + (long)n0 * ld;
             const int kv = min(BK, K - k0);
             const int nrec = ((nv - 1) * (int)ld + kv) * (int)sizeof(T);
             const __amdgpu_buffer_rsrc_t r =
-                __builtin_amdgcn_make_buffer_rsrc((void*)(base + k0), 0, nrec, 0x00020000);
+                __builtin_amdgcn_make_buffer_rsrc((void*)(slab + k0), 0, nrec, 0x00020000);
             T* dst = &Bs[kt % S][0];
 #pragma unroll
-            for (int j = 0; j < IPW; ++j) {
-                const int grp = half * IPW + j;
+            for (int j = 0; j < IPWB; ++j) {
+                const int grp = half * IPWB + j;
                 dl_load(r, dst + grp * GS, vkn + grp * gstride);
             }
         }
@@ -325,37 +348,40 @@ __global__ void __launch_bounds__(NT, 2) gemm_dl_kernel(GemmArgs<T> g) {
     for (int p = 0; p < S - 1; ++p)
         if (p < nk) issue(p);
 
-    const int ar = wm * 64 + (lane & 15);
+    const int ar = wm * 16 * WMB + (lane & 15);
     const int kq = lane >> 4;
     // B operand element (k, n = wn*64 + 16j + lane%16) in LDS
     const int bn = wn * 64 + (lane & 15);
-    const int boff = TB == 1 ? bn : (bn / CPI) * GS + (bn % CPI) * BK;
-    constexpr int BJ = TB == 1 ? 16 : (16 / CPI) * GS;  // step per j
-    constexpr int BKS = TB == 1 ? LDC : 1;              // step per k
+    const int boffC = bn, boffS = (bn / CPI) * GS + (bn % CPI) * BK;  // panel / slab layouts
+    constexpr int BJS = (16 / CPI) * GS;                // slab layout: step per j
     for (int kt = 0; kt < nk; ++kt) {
-        if (kt + S - 2 < nk) wait_vm<(S - 2) * IPW>();
-        else wait_vm<0>();
+        // this wave's loads of slice kt are done once at most the later slices' are pending
+        if (kt + S - 2 >= nk) wait_vm<0>();
+        else if (isA) wait_vm<(S - 2) * IPWA>();
+        else wait_vm<(S - 2) * IPWB>();
         bar();
         if (kt + S - 1 < nk) issue(kt + S - 1);
         const int cur = kt % S;
         const bool tail = kt * BK + BK > K;
+        const bool slab = TB == 0 || (TB == 2 && kt * BK >= n0 + BN);
+        const int boff = slab ? boffS : boffC, bj = slab ? BJS : 16, bks = slab ? 1 : LDB;
 #pragma unroll
         for (int kk = 0; kk < BK / 4; ++kk) {
             const int k = kk * 4 + kq;
-            T a[4], b[4];
+            T a[WMB], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[cur][k][ar + 16 * i];
+            for (int i = 0; i < WMB; ++i) a[i] = As[cur][k][ar + 16 * i];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[cur][boff + BJ * j + BKS * k];
+            for (int j = 0; j < 4; ++j) b[j] = Bs[cur][boff + bj * j + bks * k];
             if (tail) {
                 const bool ok = kt * BK + k < K;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = ok ? a[i] : T(0);
+                for (int i = 0; i < WMB; ++i) a[i] = ok ? a[i] : T(0);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) b[j] = ok ? b[j] : T(0);
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < WMB; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mac(a[i], b[j], acc[i][j]);
         }
@@ -363,14 +389,14 @@ __global__ void __launch_bounds__(NT, 2) gemm_dl_kernel(GemmArgs<T> g) {
 
     const T alpha = g.alpha, beta = g.beta;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < WMB; ++i) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int col = n0 + wn * 64 + j * 16 + (lane & 15);
             if (col >= N) continue;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const int row = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, t);
+                const int row = m0 + wm * 16 * WMB + i * 16 + Mfma<T>::row(lane, t);
                 if (row >= M) continue;
                 if (tri && row + g.tri_off < col) continue;
                 T* c = s.C + row + (long)col * s.ldc;
@@ -380,6 +406,16 @@ __global__ void __launch_bounds__(NT, 2) gemm_dl_kernel(GemmArgs<T> g) {
             }
         }
     }
+}
+
+// out[b][r + 128c] = sym(B)[128b + r][128b + c] from the lower triangle of B.
+template <typename T>
+__global__ void symdiag_kernel(const T* __restrict__ B, long ldb, int K, T* __restrict__ out) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;  // element of the 128 x 128 block
+    const int r = idx % BN, c = idx / BN;
+    const long gr = (long)blockIdx.y * BN + r, gc = (long)blockIdx.y * BN + c;
+    if (gr >= K || gc >= K) return;
+    out[(long)blockIdx.y * BN * BN + idx] = r >= c ? B[gr + gc * ldb] : B[gc + gr * ldb];
 }
 
 template <typename T>
@@ -416,18 +452,24 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
     const bool kn_offsets_fit = (long)BN * g.ldb * (long)sizeof(T) < (1L << 31);
     if (small) {
         grid.x = (maxM + 63) / 64;
-        if (g.transb == 0)
+        if (g.transb == 0 && kn_offsets_fit)
+            hipLaunchKernelGGL((gemm_dl_kernel<T, 0, kStages, 64>), grid, dim3(NT), 0, stream, g);
+        else if (g.transb == 2 && g.symdiag && kn_offsets_fit)
+            hipLaunchKernelGGL((gemm_dl_kernel<T, 2, kStages, 64>), grid, dim3(NT), 0, stream, g);
+        else if (g.transb == 0)
             hipLaunchKernelGGL((gemm_kernel<T, 0, 2>), grid, dim3(NT), 0, stream, g);
         else if (g.transb == 1)
-            hipLaunchKernelGGL((gemm_kernel<T, 1, 2>), grid, dim3(NT), 0, stream, g);
+            hipLaunchKernelGGL((gemm_dl_kernel<T, 1, kStages, 64>), grid, dim3(NT), 0, stream, g);
         else
             hipLaunchKernelGGL((gemm_kernel<T, 2, 2>), grid, dim3(NT), 0, stream, g);
     } else if (g.transb == 0 && kn_offsets_fit)
-        hipLaunchKernelGGL((gemm_dl_kernel<T, 0, kStages>), grid, dim3(NT), 0, stream, g);
+        hipLaunchKernelGGL((gemm_dl_kernel<T, 0, kStages, 128>), grid, dim3(NT), 0, stream, g);
+    else if (g.transb == 2 && g.symdiag && kn_offsets_fit)
+        hipLaunchKernelGGL((gemm_dl_kernel<T, 2, kStages, 128>), grid, dim3(NT), 0, stream, g);
     else if (g.transb == 0)
         hipLaunchKernelGGL((gemm_kernel<T, 0, 4>), grid, dim3(NT), 0, stream, g);
     else if (g.transb == 1)
-        hipLaunchKernelGGL((gemm_dl_kernel<T, 1, kStages>), grid, dim3(NT), 0, stream, g);
+        hipLaunchKernelGGL((gemm_dl_kernel<T, 1, kStages, 128>), grid, dim3(NT), 0, stream, g);
     else
         hipLaunchKernelGGL((gemm_kernel<T, 2, 4>), grid, dim3(NT), 0, stream, g);
     MOAO_HIP_TRY(hipGetLastError());
@@ -438,6 +480,17 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
 
 int gemm_f64(const GemmArgs<double>& a, hipStream_t stream) { return launch(a, stream); }
 int gemm_f32(const GemmArgs<float>& a, hipStream_t stream) { return launch(a, stream); }
+
+size_t gemm_symdiag_elems(long K) { return (size_t)((K + BN - 1) / BN) * BN * BN; }
+
+int gemm_symdiag(const double* B, long ldb, int K, double* out, hipStream_t stream) {
+    if (K <= 0) return MOAO_OK;
+    ProfScope ps(kProfOther, 0.0, stream);
+    hipLaunchKernelGGL(symdiag_kernel<double>, dim3(BN * BN / 256, (K + BN - 1) / BN), dim3(256),
+                       0, stream, B, ldb, K, out);
+    MOAO_HIP_TRY(hipGetLastError());
+    return MOAO_OK;
+}
 
 }  // namespace moao
 
@@ -455,7 +508,23 @@ int moao_hip_dgemm_dev(int transb, int M, int N, int K, double alpha, const doub
         return MOAO_ERR_ARG;
     }
     auto a = moao::gemm_args<double>(M, N, K, alpha, A, lda, B, ldb, transb, beta, C, ldc);
-    return moao::gemm_f64(a, moao::pick_stream(stream));
+    hipStream_t s = moao::pick_stream(stream);
+    if (transb != 2 || K == 0) return moao::gemm_f64(a, s);
+    // symmetric B: its diagonal blocks in stream-ordered scratch (register-staged kernel if
+    // that allocation fails)
+    double* sd = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&sd), sizeof(double) * moao::gemm_symdiag_elems(K),
+                       s) != hipSuccess) {
+        (void)hipGetLastError();
+        return moao::gemm_f64(a, s);
+    }
+    int rc = moao::gemm_symdiag(B, ldb, K, sd, s);
+    if (rc == MOAO_OK) {
+        a.symdiag = sd;
+        rc = moao::gemm_f64(a, s);
+    }
+    (void)hipFreeAsync(sd, s);
+    return rc;
 }
 
 int moao_hip_sgemm_dev(int transb, int M, int N, int K, float alpha, const float* A, long lda,

@@ -41,7 +41,17 @@ struct GemmArgs {
     int tri;      // segment 0 lower-trapezoid restriction
     int tri_off;
     T alpha, beta;
+    // transb = 2 only, optional: full symmetric copies of B's 128 x 128 diagonal blocks
+    // (gemm_symdiag); with it the launch runs on the direct-to-LDS kernel.
+    const T* symdiag;
 };
+
+// Elements of the gemm_symdiag copy of a K x K matrix.
+size_t gemm_symdiag_elems(long K);
+// out[b] (128 x 128, leading dimension 128) = the full symmetric diagonal block b of the K x K
+// matrix whose lower triangle B holds; what a transb = 2 launch reads where its K slices cross
+// the diagonal.
+int gemm_symdiag(const double* B, long ldb, int K, double* out, hipStream_t stream);
 
 // Launches on `stream`; returns a MOAO_* status (argument checks only, launch is async).
 int gemm_f64(const GemmArgs<double>& a, hipStream_t stream);
@@ -66,6 +76,7 @@ inline GemmArgs<T> gemm_args(int M, int N, int K, T alpha, const T* A, long lda,
     g.tri_off = 0;
     g.alpha = alpha;
     g.beta = beta;
+    g.symdiag = nullptr;
     return g;
 }
 

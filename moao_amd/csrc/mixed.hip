@@ -197,6 +197,7 @@ static size_t mixed_layout(long n, long m, int p, std::vector<size_t>* offs) {
     take(sizeof(double) * kNormBlocks);
     take(sizeof(int) * n);
     take(sizeof(int) * 4);
+    take(sizeof(double) * gemm_symdiag_elems(n));  // Cmm's symmetric diagonal blocks
     return off;
 }
 
@@ -225,6 +226,7 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
               (double*)(b + offs[6]), (double*)(b + offs[7]), (double*)(b + offs[8]),
               (double*)(b + offs[9]), (double*)(b + offs[10]), (double*)(b + offs[11]),
               (int*)(b + offs[12]), (int*)(b + offs[13])};
+    double* const symd = (double*)(b + offs[14]);
     const int N = (int)n, M = (int)m;
 
     auto norm = [&](const double* a, long lda, int rows, int cols, double* out) -> int {
@@ -344,7 +346,8 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
         std::fprintf(stderr, "[mixed] |Z|=%.6e |Z2|=%.6e |R0|=%.6e\n", a, b2, c2);
     }
 
-    // --- 4. refinement
+    // --- 4. refinement (the residual's symmetric operand Cmm feeds the direct-to-LDS GEMM)
+    if (int rc = gemm_symdiag(Cmm, ldCmm, N, symd, s)) return rc;
     double rel = 0;
     int it = 0;
     int status = MOAO_OK;
@@ -352,6 +355,7 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
         MOAO_HIP_TRY(hipMemcpyAsync(w.E, w.Ctm0, sizeof(double) * m * n, hipMemcpyDeviceToDevice,
                                     s));
         auto g = gemm_args<double>(M, N, N, -1.0, Ctm, ldCtm, Cmm, ldCmm, 2, 1.0, w.E, m);
+        g.symdiag = symd;
         if (int rc = gemm_f64(g, s)) { status = rc; break; }
         double en = 0;
         if (int rc = norm(w.E, m, M, N, &en)) { status = rc; break; }
