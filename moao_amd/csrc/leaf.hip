@@ -11,12 +11,12 @@
 // processed by 16-column blocks:
 //   load:   batched global loads (16 in flight per lane) before the LDS stores;
 //   factor: [wave 0] 16x16 diagonal block in registers (readlane broadcasts, 1/sqrt of each
-//           pivot by a refined hardware estimate, kept for the later divisions); [all] rows
-//           below by right-looking substitution (one lane per row); [all] rank-16 update of
-//           the trailing lower triangle on f64 MFMA;
-//   invert: block columns from last to first: [wave 0] D = inv(L_jj) by substitution;
-//           [all] T = inv(L)[below, below] * L[below, j] and column j of inv(L) = -T * D, both
-//           on f64 MFMA, 16-row tiles per wave.
+//           pivot by a refined hardware estimate), then D = inv(L_bb) by right-looking
+//           substitution, kept in the unused block above the diagonal; [all] rows below as
+//           A * D^T and the rank-16 update of the trailing lower triangle, both on f64 MFMA;
+//   invert: block columns from last to first: [all] T = inv(L)[below, below] * L[below, j]
+//           and column j of inv(L) = -T * D, both on f64 MFMA, 16-row tiles per wave;
+//   store:  L and inv(L) with 16 LDS reads per lane in flight before the global stores.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -140,27 +140,51 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
 #pragma unroll
                 for (int k = 0; k < kB; ++k)
                     if (k <= i) S_(kb + i, kb + k) = a[k];
+            // D = inv(L_bb) (same wave, so the LDS writes above are visible): lane c builds
+            // column c right-looking from e_c; rows at or past bs are zero padding.  D is kept
+            // in the unused block above the diagonal, S_(kb + r, kb + kB + c), or in D[] for
+            // the last block; the rows below and the inverse phase both use it.
+            if (lane < kB) {
+                const int c = lane;
+                double x[kB];
+#pragma unroll
+                for (int r = 0; r < kB; ++r) x[r] = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+                for (int r = 0; r < kB; ++r) {
+                    x[r] = r < bs ? x[r] * rdiag[kb + r] : 0.0;
+#pragma unroll
+                    for (int r2 = r + 1; r2 < kB; ++r2) x[r2] -= S_(kb + r2, kb + r) * x[r];
+                }
+                if (kb + kB < kLeaf) {
+#pragma unroll
+                    for (int r = 0; r < kB; ++r) S_(kb + r, kb + kB + c) = x[r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < kB; ++r) D[r * kB + c] = x[r];
+                }
+            }
         }
         __syncthreads();
         if (fail) return;
-        // rows below the diagonal block: x * L_bb^T = a  (forward substitution per row)
+        // rows below the diagonal block: L[r, kb:kb+16] = A[r, kb:kb+16] * D^T on MFMA, a
+        // 16-row tile per wave turn, in place (bs is 16 whenever rows remain below the block)
         const int rest = nb - kb - bs;
-        // (bs is 16 whenever rows remain below the block).  Right-looking: once x[c] is known
-        // every later x[c2] takes its term at once, so each step is one multiply followed by
-        // independent FMAs instead of a dot-product chain.
-        if (tid < rest) {
-            const int r = kb + kB + tid;
-            double x[kB];
+        if (rest > 0) {
+            const bool up = kb + kB < kLeaf;
+            const double* Dp = up ? &S_(kb, kb + kB) : D;  // D element (r, c) at Dp[r*dr + c*dc]
+            const int dr = up ? 1 : kB, dc = up ? LD : 1;
+            const int ht = (rest + 15) / 16;
+            for (int i = wave; i < ht; i += NTL / 64) {
+                const int r0 = kb + kB + 16 * i;
+                f64x4 acc = {0, 0, 0, 0};
+                // Y element (k, c) = D^T(k, c) = D(c, k)
+                acc = mfma_block(&S_(0, kb), LD, r0, Dp, dc, dr, 0, 0, kB / 4, acc, lane);
 #pragma unroll
-            for (int c = 0; c < kB; ++c) x[c] = S_(r, kb + c);
-#pragma unroll
-            for (int c = 0; c < kB; ++c) {
-                x[c] *= rdiag[kb + c];
-#pragma unroll
-                for (int c2 = c + 1; c2 < kB; ++c2) x[c2] -= x[c] * S_(kb + c2, kb + c);
+                for (int e = 0; e < 4; ++e) {
+                    const int r = r0 + (lane >> 4) + 4 * e;
+                    if (r < nb) S_(r, kb + (lane & 15)) = acc[e];
+                }
             }
-#pragma unroll
-            for (int c = 0; c < kB; ++c) S_(r, kb + c) = x[c];
         }
         __syncthreads();
         // trailing update of the lower triangle on MFMA: 16 x 16 tiles (ti >= tj) of the
@@ -187,9 +211,23 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
         }
         __syncthreads();
     }
-    // L back to global (lower part only).
-    for (int c = wave; c < nb; c += NTL / 64)
-        for (int r = c + lane; r < nb; r += 64) A[r + (long)c * lda] = (TS)S_(r, c);
+    // L back to global (lower part only), 16 LDS reads per lane in flight before the stores.
+    for (int cb = wave; cb < nb; cb += 4 * 8) {
+        double v0[8], v1[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = min(cb + 4 * q, kLeaf - 1);
+            v0[q] = S_(lane, c);
+            v1[q] = S_(lane + 64, c);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = cb + 4 * q;
+            if (c >= nb) break;
+            if (lane >= c && lane < nb) A[lane + (long)c * lda] = (TS)v0[q];
+            if (lane + 64 >= c && lane + 64 < nb) A[lane + 64 + (long)c * lda] = (TS)v1[q];
+        }
+    }
 
     // ------------------------------------------------------------------ inverse
     const int nblocks = (nb + kB - 1) / kB;
@@ -197,22 +235,10 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
         const int c0 = jb * kB, bs = min(kB, nb - c0);
         const int below = nb - c0 - bs;
         const int kbeg = c0 + bs;
-        // D = inv(L_jj): lane c computes column c by forward substitution
-        // (right-looking from e_c: rows above c stay 0; rows at or past bs are zero padding)
-        if (wave == 0 && lane < kB) {
-            const int c = lane;
-            double x[kB];
-#pragma unroll
-            for (int r = 0; r < kB; ++r) x[r] = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-            for (int r = 0; r < kB; ++r) {
-                x[r] = r < bs ? x[r] * rdiag[c0 + r] : 0.0;
-#pragma unroll
-                for (int r2 = r + 1; r2 < kB; ++r2) x[r2] -= S_(c0 + r2, c0 + r) * x[r];
-            }
-#pragma unroll
-            for (int r = 0; r < kB; ++r) D[r * kB + c] = x[r];  // D[r][c]
-        }
+        // D = inv(L_jj), computed during the factorization: copy it out of its storage above
+        // the diagonal (the last block left it in D[] already); used after the next barrier
+        if (c0 + kB < kLeaf)
+            for (int t = tid; t < kB * kB; t += NTL) D[t] = S_(c0 + t / kB, c0 + kB + t % kB);
         // T = inv(L)[below, below] * L[below, c0:c0+16] on MFMA, a 16-row tile per wave turn;
         // inv(L) of the trailing part is lower triangular (zero above, zero padding below nb),
         // so row tile i needs k < 16 (i + 1) only
@@ -242,9 +268,21 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
         }
         __syncthreads();
     }
-    for (int c = wave; c < kLeaf; c += NTL / 64)
-        for (int r = lane; r < kLeaf; r += 64)
-            inv[r + (long)c * kLeaf] = (TS)((r < nb && c < nb && r >= c) ? S_(r, c) : 0.0);
+    for (int cb = wave; cb < kLeaf; cb += 4 * 8) {
+        double v0[8], v1[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            v0[q] = S_(lane, cb + 4 * q);
+            v1[q] = S_(lane + 64, cb + 4 * q);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = cb + 4 * q;
+            inv[lane + (long)c * kLeaf] = (TS)((lane < nb && c < nb && lane >= c) ? v0[q] : 0.0);
+            inv[lane + 64 + (long)c * kLeaf] =
+                (TS)((lane + 64 < nb && c < nb && lane + 64 >= c) ? v1[q] : 0.0);
+        }
+    }
 }
 
 #undef S_
