@@ -88,9 +88,9 @@ struct MatcovArgs {
     int truth_nsub;         // Nsubap of the truth sensor
 };
 
-__device__ __forceinline__ double vk_series(double x) {
+// cx = cbrt(x), shared with vk_asymptote: lanes of one wave often take both branches.
+__device__ __forceinline__ double vk_series(double x, double cx) {
     // x^(5/3) as x * cbrt(x)^2: a cube root costs a fraction of a general pow
-    const double cx = cbrt(x);
     const double x2a = x * cx * cx;
     const double x22 = x * x / 4.;
     double xn = 0.5;
@@ -105,15 +105,16 @@ __device__ __forceinline__ double vk_series(double x) {
     return s;
 }
 
-__device__ __forceinline__ double vk_asymptote(double x) {
+__device__ __forceinline__ double vk_asymptote(double x, double cx) {
     const double ix = 1. / x;
-    return kK2 - kK3 * exp(-x) * cbrt(x) * (1.0 + ix * (kA1 + ix * (kA2 + ix * kA3)));
+    return kK2 - kK3 * exp(-x) * cx * (1.0 + ix * (kA1 + ix * (kA2 + ix * kA3)));
 }
 
 __device__ __forceinline__ double vk_dphi(double x, double y, double lk, double amp) {
     const double r = sqrt(x * x + y * y);
     const double q = lk * r;
-    double res = (q > kXlim) ? vk_asymptote(q) : -vk_series(q);
+    const double cq = cbrt(q);
+    double res = (q > kXlim) ? vk_asymptote(q, cq) : -vk_series(q, cq);
     res *= amp;
     return res;
 }
