@@ -122,11 +122,21 @@ __device__ __forceinline__ double vk_dphi(double x, double y, double lk, double 
 __device__ __forceinline__ double slope_cov(int kind, double du, double dv, double s1, double s2,
                                             double lk, double amp) {
     const double ac = s1 - s2, ad = s1 + s2, bc = -ad, bd = -ac;
+    // Equal sizes (s1 == s2, most WFS pairs): ac = +0 and bd = -0, so the first and last
+    // terms share one argument exactly; evaluating it once keeps every operation and its order.
     switch (kind) {
         case 0:
+            if (ac == 0.0) {
+                const double d0 = vk_dphi(du, dv, lk, amp);
+                return -d0 + vk_dphi(du + ad, dv, lk, amp) + vk_dphi(du + bc, dv, lk, amp) - d0;
+            }
             return -vk_dphi(du + ac, dv, lk, amp) + vk_dphi(du + ad, dv, lk, amp) +
                    vk_dphi(du + bc, dv, lk, amp) - vk_dphi(du + bd, dv, lk, amp);
         case 3:
+            if (ac == 0.0) {
+                const double d0 = vk_dphi(du, dv, lk, amp);
+                return -d0 + vk_dphi(du, dv + ad, lk, amp) + vk_dphi(du, dv + bc, lk, amp) - d0;
+            }
             return -vk_dphi(du, dv + ac, lk, amp) + vk_dphi(du, dv + ad, lk, amp) +
                    vk_dphi(du, dv + bc, lk, amp) - vk_dphi(du, dv + bd, lk, amp);
         case 1:
