@@ -225,12 +225,11 @@ __global__ void __launch_bounds__(NT, WMB == 4 ? 2 : 4) gemm_kernel(GemmArgs<T> 
     }
 }
 
-
 // Direct-to-LDS kernel: S slices in flight.  A and op(B) = B^T tiles are column panels of 128
-// rows: one descriptor per column, so the descriptor's range check zero-fills rows past the
-// matrix.  op(B) = B (K x N) tiles are
-// BK-row slabs of 128 columns: one descriptor per slab whose range ends at the slab's last
-// element, 4 (FP64) or 8 (FP32) columns per 256-byte load, column groups padded in LDS.
+// (A: TM) rows: one descriptor per column, so the descriptor's range check zero-fills rows past
+// the matrix.  op(B) = B (K x N) tiles are BK-row slabs of 128 columns: one descriptor per slab
+// whose range ends at the slab's last element, 4 (FP64) or 8 (FP32) columns per 256-byte load,
+// column groups padded in LDS.  Symmetric B switches between the two per slice.
 // The LDS pointer passes through an empty asm so the compiler cannot trace it to one LDS array:
 // with that provenance it would put a vmcnt(0) before every later read of the same array and
 // serialize the pipeline.  Ordering is explicit instead: wait_vm + bar before each slice's reads.
@@ -240,11 +239,15 @@ __device__ __forceinline__ void dl_load(__amdgpu_buffer_rsrc_t r, T* lds, int vo
     asm("" : "+s"(l));  // the 32-bit LDS address, so no generic-to-LDS conversion follows
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, l, 4, voff, 0, 0, 0);
 }
+// s_waitcnt vmcnt(VM) alone (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4 and lgkmcnt
+// 11:8 left at their maxima).
 template <int VM>
 __device__ __forceinline__ void wait_vm() {
     static_assert(VM >= 0 && VM < 64, "vmcnt range");
     __builtin_amdgcn_s_waitcnt((VM & 15) | ((VM >> 4) << 14) | (7 << 4) | (15 << 8));
 }
+// Workgroup barrier without the memory fence of __syncthreads (which would also drain vmcnt);
+// the empty asm statements keep the compiler from moving LDS accesses across it.
 __device__ __forceinline__ void bar() {
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
