@@ -7,17 +7,19 @@
 // and r = l/16 + 4t for f64.
 //
 // Two kernels share that structure:
-//  * gemm_dl_kernel (op(B) = B or B^T, 128-row tiles): buffer loads write LDS directly
+//  * gemm_dl_kernel (every launch with op(B) = B or B^T, and symmetric B when the caller
+//    supplies full copies of its diagonal blocks): buffer loads write LDS directly
 //    (buffer_load_dword ... lds), so slices need no register staging; kStages slices are in
 //    flight and each slice costs one vmcnt wait and one plain s_barrier.  The buffer
 //    descriptors' range checks zero-fill what lies past the matrix (the range includes the
 //    scalar offset on gfx950, so every column gets a descriptor of its own).  Measured
-//    16384^3 FP64: 74.8 TF/s (95% of the MFMA peak).
-//  * gemm_kernel (symmetric B, and 64-row tiles): register-staged, double-buffered LDS, the
-//    next slice's global loads issued before the current slice's MFMAs.  A launch with fewer
-//    than two 128-row tiles per CU (panel solves of the leaves, narrow updates) uses 64-row
-//    tiles (each wave 32x64, up to 4 workgroups per CU), which doubles the workgroups that
-//    fill the device.
+//    16384^3 FP64: 74 TF/s (94% of the MFMA peak).
+//  * gemm_kernel (fallback: symmetric B without diagonal copies, K x N slabs whose byte
+//    offsets would overflow 32 bits): register-staged, double-buffered LDS, the next slice's
+//    global loads issued before the current slice's MFMAs.
+// A launch with fewer than two 128-row tiles per CU (panel solves of the leaves, narrow
+// updates) uses 64-row tiles (each wave 32x64, up to 4 workgroups per CU), which doubles the
+// workgroups that fill the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
