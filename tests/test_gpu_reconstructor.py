@@ -81,6 +81,24 @@ def test_random_spd_sizes():
         _check(C, B, 1e-11)
 
 
+@pytest.mark.parametrize("n,m", [(9000, 1500), (20480, 2560)])
+def test_large_sizes(n, m):
+    """Sizes whose top-level updates launch >= 512 tiles of 128 rows: the trapezoid updates
+    with the Ctm rows as a second segment and the K x N backward updates run on the 128-row
+    kernels (the configuration of the n = 40k workload), checked like the small ones."""
+    print(f"[recon] large random SPD n={n} m={m}", flush=True)
+    rng = np.random.default_rng(n)
+    # symmetric with a semicircle spectrum in [-1, 1], shifted: SPD with eigenvalues in [2, 4]
+    G = rng.standard_normal((n, n))
+    C = np.tril(G + G.T)
+    del G
+    C *= 0.25 / np.sqrt(n)
+    C[np.diag_indices(n)] += 3.0
+    C = np.asfortranarray(C)
+    B = np.asfortranarray(rng.standard_normal((m, n)))
+    _check(C, B, 1e-10)
+
+
 def test_leading_dims_and_targets():
     """ldCmm > n, ldCtm > m and two targets stored n*m apart (reconstructor.cpp:43)."""
     print("[recon] leading dims + 2 targets", flush=True)
