@@ -16,8 +16,9 @@ pytestmark = pytest.mark.gpu
 
 CASES = [(128, 128, 16, 1, 0.0), (130, 77, 33, 1, 1.0), (300, 129, 200, 0, -1.0),
          (1, 1, 1, 0, 0.5), (257, 383, 700, 1, 1.0), (64, 500, 5, 0, 0.0),
-         # >= 512 tiles of 128 rows: the 128-row kernels
-         (4133, 3900, 300, 1, 1.0), (4133, 3900, 301, 0, -1.0),
+         # >= 512 tiles of 128 rows: the 128-row kernels; 561 tiles leave a short last wave
+         # whose tiles run as K chunks plus a reduction
+         (4133, 3900, 300, 1, 1.0), (4133, 3900, 301, 0, -1.0), (4133, 2100, 301, 0, 1.0),
          # symmetric B (K == N)
          (50, 300, 300, 2, 1.0), (257, 129, 129, 2, 0.0), (7, 16, 16, 2, 1.0),
          (200, 77, 77, 2, -1.0), (4133, 2100, 2100, 2, 0.5)]
@@ -60,7 +61,7 @@ def test_gemm_matches_numpy(M, N, K, transb, beta, dtype):
     assert np.array_equal(out[M:], C[M:])
 
 
-@pytest.mark.parametrize("M,N,K,transb", [(4133, 3900, 300, 1), (4133, 3900, 301, 0),
+@pytest.mark.parametrize("M,N,K,transb", [(4133, 3900, 300, 1), (4133, 2100, 301, 0),
                                           (4133, 2100, 2100, 2), (300, 129, 203, 1),
                                           (300, 129, 203, 0), (200, 77, 77, 2)])
 def test_gemm_reads_only_the_operands(M, N, K, transb):
