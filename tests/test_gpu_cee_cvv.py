@@ -70,3 +70,36 @@ def test_symmetric_b_gemm_mode():
         dgemm_dev(2, M, K, K, 1.0, dA.ptr, M, dB.ptr, K, 0.0, dC.ptr, M)
         ref = A @ S
         assert np.max(np.abs(dC.to_host() - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_cee_cvv_large_synthetic():
+    """n = 9000: step 2 (R sym(Cmm), 12 x 71 tiles of 128 rows) runs on the symmetric
+    direct-to-LDS kernel with Cmm's full diagonal-block copies; the whole chain is checked
+    against the oracle on synthetic operands (random SPD Cmm with garbage above its diagonal,
+    random symmetric Ctt, random Ctm and R: no cancellation, so FP64 rounding alone, 1e-11)."""
+    n, m, nact = 9000, 1500, 600
+    print(f"[cee] large synthetic n={n} m={m} nact={nact}", flush=True)
+    rng = np.random.default_rng(21)
+    G = rng.standard_normal((n, n))
+    C = np.tril(G + G.T) * (0.25 / np.sqrt(n))
+    del G
+    C[np.diag_indices(n)] += 3.0
+    C[np.triu_indices(n, 1)] = 1e300  # never read (dsymm 'L')
+    C = np.asfortranarray(C)
+    T = rng.standard_normal((m, m))
+    Ctt = np.asfortranarray(T + T.T)
+    Ctm = np.asfortranarray(rng.standard_normal((m, n)))
+    R = np.asfortranarray(rng.standard_normal((m, n)) * 0.01)
+    Dx = np.asfortranarray(rng.standard_normal((nact, m)))
+    Cl = np.asfortranarray(np.tril(C))
+    Cpp_o, Cpm_o = Ctt.copy(order="F"), Ctm.copy(order="F")
+    Cee_o, Cvv_o = lp.cee_cvv(Cl, Cpp_o, Cpm_o, R, Dx)
+    Cpp, Cpm = Ctt.copy(order="F"), Ctm.copy(order="F")
+    Cee = np.zeros((m, m), order="F")
+    Cvv = np.zeros((nact, nact), order="F")
+    Tmp = np.zeros((nact, m), order="F")
+    cee_cvv(n, m, nact, C, Cpp, Cpm, R, Dx, Cee, Cvv, Tmp)
+    e_cee, e_cvv, e_cpm = rel_fro(Cee, Cee_o), rel_fro(Cvv, Cvv_o), rel_fro(Cpm, Cpm_o)
+    print(f"    Cee {e_cee:.2e} Cvv {e_cvv:.2e} Cpm {e_cpm:.2e}", flush=True)
+    assert e_cee <= 1e-11 and e_cvv <= 1e-11 and e_cpm <= 1e-12
+    assert np.all(np.isfinite(Cee)) and np.all(np.isfinite(Cvv))
