@@ -50,6 +50,16 @@ def test_mixed_small_workload(tmp_path):
     assert res <= 1e-11 and err <= 1e-6
 
 
+def test_mixed_elt_workload(tmp_path):
+    """n = 18,688 (the 40x40, 6 LGS + 2 NGS system): the FP32 factorization and solves run on
+    the 128-row FP32 kernels and every FP64 residual on the symmetric direct-to-LDS kernel."""
+    print("[mixed] seeded elt workload (n=18688)", flush=True)
+    s, atms = systems.make_system(str(tmp_path), "elt")
+    err, res, its = _run(Tomo(s, atms[0]))
+    print(f"    err={err:.2e} resid={res:.2e} steps={its}", flush=True)
+    assert res <= 1e-11 and err <= 1e-6
+
+
 def test_mixed_without_lowrank_split(tmp_path):
     """No low-rank description: FP32 factorization of Cmm itself (small n, still SPD in FP32)."""
     print("[mixed] no low-rank split", flush=True)
