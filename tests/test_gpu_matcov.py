@@ -83,3 +83,43 @@ def test_cmm_is_spd_and_symmetric_blocks():
     full = np.tril(c) + np.tril(c, -1).T
     w = np.linalg.eigvalsh(full)
     assert w.min() > 0
+
+
+def test_elt_workload_tiles(tmp_path):
+    """The n = 18,688 system (40x40 subapertures, 6 LGS + 2 NGS, 10 layers: elongated LGS next to
+    NGS, layers some WFS do not see): 256 x 256 tiles of Cmm (on and off the diagonal, the
+    last rows and columns), Ctm and Ctt against the oracle.  As for the whole fixture matrices,
+    elements must agree to 1e-12 of the matrix's largest entry (its diagonal: Cmm's own for Cmm,
+    the truth sensor's Ctt for Ctm and Ctt).  A per-tile relative norm is not a meaningful bar
+    away from the diagonal: the ulp-level device/libm differences of the structure function
+    stay at ~1e-14 of the diagonal scale everywhere (measured), while far-apart WFS pairs have
+    covariances 1e2-1e5 times smaller than that scale."""
+    from moao_amd import systems
+    s, a = systems.make_system(str(tmp_path), "elt")
+    t, o = Tomo(s, a[0]), OTomo.from_files(s, a[0])
+    n, m = t.nmeas(), t.nmeasts()
+    print(f"[matcov] elt workload n={n} m={m}", flush=True)
+    t.set_target(0)
+    o.set_target(0)
+    T = 256
+
+    def tile(typ, xo, yo):
+        buf = np.full((T, T), -1.0, order="F")
+        gm.matcov_tile(buf, T, T, xo, yo, T, t, typ)
+        ref = np.full((T, T), -1.0, order="F")
+        om.matcov_tile(ref, T, T, xo, yo, T, o, typ)
+        if typ == 1:  # only the lower part (row >= col) is generated
+            keep = (np.arange(T)[:, None] + xo) >= (np.arange(T)[None, :] + yo)
+            assert np.all(buf[~keep] == -1.0)
+            buf, ref = np.where(keep, buf, 0.0), np.where(keep, ref, 0.0)
+        return buf, ref
+
+    scale = {1: np.max(np.abs(tile(1, 0, 0)[1]))}
+    scale[3] = scale[4] = np.max(np.abs(tile(4, 0, 0)[1]))
+    tiles = [(1, 0, 0), (1, 9000, 4000), (1, n - T, n - T), (1, 12000, 11900), (1, n - T, 0),
+             (1, 3000, 2000), (3, 1000, 15000), (3, m - T, n - T), (4, 0, 0), (4, 2000, 1000)]
+    for typ, xo, yo in tiles:
+        g, r = tile(typ, xo, yo)
+        err = np.max(np.abs(g - r)) / scale[typ]
+        print(f"    type {typ} tile at ({xo}, {yo}): {err:.2e} of the diagonal scale", flush=True)
+        assert err <= 1e-12
