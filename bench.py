@@ -423,7 +423,13 @@ def main():
                         "speedup_vs_fp64": (results["fp64"]["total"] / mr["total"]
                                             if "fp64" in results else None),
                         "stages_ms_per_solve": {k: v["ms"] / args.steps
-                                                for k, v in mr["prof"].items() if v["count"]}}
+                                                for k, v in mr["prof"].items() if v["count"]},
+                        # MFMA rate of each GEMM precision: algorithmic flops / summed kernel time
+                        "gemm_tflops": {k: {"achieved": v["work"] / (v["ms"] * 1e-3) / 1e12,
+                                            "peak": (PEAK_FP64 if k == "gemm_f64" else 157.3e12)
+                                            / 1e12}
+                                        for k, v in mr["prof"].items()
+                                        if k.startswith("gemm_") and v["count"] and v["ms"] > 0}}
     if not args.no_stages and "fp64" in results:
         # R of the last FP64 step is resident in dCtm; Cmm0 is the regenerated Cmm
         out["stages"] = stages(tomo, n, m, dCmm0, dCtm0, dCtm)
