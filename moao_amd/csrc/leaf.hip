@@ -51,19 +51,39 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
     return r;
 }
 
-// 16 x 16 block product on MFMA: acc += X[r0:r0+16, k0:k0+4*nk4] * Y[k0:k0+4*nk4, c0:c0+16] with
-// X element (r, k) at X[r + k*ldx] and Y element (k, c) at Y[yk*k + yc*c] (LDS).  Lane l
-// supplies X[r0 + l%16][k] and Y[k][c0 + l%16] for k = k0 + 4q + l/16; accumulator item t of
-// lane l is row l/16 + 4t, column l%16.
+// 16 x 16 block product on MFMA: acc += X[r0:r0+16, k0:k0+16*nc] * Y[k0:k0+16*nc, c0:c0+16]
+// (nc <= MAXC chunks of 16) with X element (r, k) at X[r + k*ldx] and Y element (k, c) at
+// Y[yk*k + yc*c] (LDS).  Lane l supplies X[r0 + l%16][k] and Y[k][c0 + l%16] for
+// k = k0 + 4s + l/16; accumulator item t of lane l is row l/16 + 4t, column l%16.  The loop is
+// unrolled and the loads of chunk q+1 are issued before the four MFMAs of chunk q, so the LDS
+// latency is paid once per call rather than per MFMA; the accumulation order is k ascending.
+template <int MAXC>
 __device__ __forceinline__ f64x4 mfma_block(const double* X, int ldx, int r0, const double* Y,
-                                             int yk, int yc, int c0, int k0, int nk4, f64x4 acc,
+                                             int yk, int yc, int c0, int k0, int nc, f64x4 acc,
                                              int lane) {
     const int li = lane & 15, lk = lane >> 4;
-    for (int q = 0; q < nk4; ++q) {
-        const int k = k0 + 4 * q + lk;
-        const double a = X[(r0 + li) + k * ldx];
-        const double b = Y[yk * k + yc * (c0 + li)];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    const double* xp = X + (r0 + li) + (k0 + lk) * ldx;  // element (r0 + li, k0 + lk)
+    const double* yp = Y + yk * (k0 + lk) + yc * (c0 + li);
+    double a[2][4], b[2][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        a[0][s] = xp[4 * s * ldx];
+        b[0][s] = yp[4 * s * yk];
+    }
+#pragma unroll
+    for (int q = 0; q < MAXC; ++q) {
+        if (q >= nc) break;
+        const int cur = q & 1;
+        if (q + 1 < nc) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                a[cur ^ 1][s] = xp[(16 * (q + 1) + 4 * s) * ldx];
+                b[cur ^ 1][s] = yp[(16 * (q + 1) + 4 * s) * yk];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][s], b[cur][s], acc, 0, 0, 0);
     }
     return acc;
 }
@@ -77,7 +97,8 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
     __shared__ double rdiag[kLeaf];          // 1 / L_jj
     __shared__ int fail;
     if (*(volatile int*)info != 0) return;  // an earlier leaf failed: R is not computed
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar branches
     if (tid == 0) fail = 0;
     // Load the lower triangle, zero-padded to kLeaf x kLeaf.  A wave owns columns
     // wave, wave+4, ...; each batch issues 16 independent loads per lane before any LDS store
@@ -178,7 +199,7 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
                 const int r0 = kb + kB + 16 * i;
                 f64x4 acc = {0, 0, 0, 0};
                 // Y element (k, c) = D^T(k, c) = D(c, k)
-                acc = mfma_block(&S_(0, kb), LD, r0, Dp, dc, dr, 0, 0, kB / 4, acc, lane);
+                acc = mfma_block<1>(&S_(0, kb), LD, r0, Dp, dc, dr, 0, 0, 1, acc, lane);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int r = r0 + (lane >> 4) + 4 * e;
@@ -199,8 +220,8 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
                 const int tj = t - acc0;
                 f64x4 acc = {0, 0, 0, 0};
                 // P[r][k] = S_(r, kb + k); B operand is P^T: element (k, c) = S_(c, kb + k)
-                acc = mfma_block(&S_(0, kb), LD, base + 16 * ti, &S_(0, kb), LD, 1, base + 16 * tj,
-                                 0, kB / 4, acc, lane);
+                acc = mfma_block<1>(&S_(0, kb), LD, base + 16 * ti, &S_(0, kb), LD, 1,
+                                    base + 16 * tj, 0, 1, acc, lane);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int r = base + 16 * ti + (lane >> 4) + 4 * e;
@@ -241,12 +262,15 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
             for (int t = tid; t < kB * kB; t += NTL) D[t] = S_(c0 + t / kB, c0 + kB + t % kB);
         // T = inv(L)[below, below] * L[below, c0:c0+16] on MFMA, a 16-row tile per wave turn;
         // inv(L) of the trailing part is lower triangular (zero above, zero padding below nb),
-        // so row tile i needs k < 16 (i + 1) only
+        // so row tile i needs k < 16 (i + 1) only.  Tiles go to the waves in snake order
+        // (0 1 2 3, then 7 6 5 4), which evens out the longer products of the later tiles.
         const int nti = (below + 15) / 16;
-        for (int i = wave; i < nti; i += NTL / 64) {
+        for (int r = 0; 4 * r < nti; ++r) {
+            const int i = (r & 1) ? 4 * r + 3 - wave : 4 * r + wave;
+            if (i >= nti) continue;
             f64x4 acc = {0, 0, 0, 0};
-            acc = mfma_block(&S_(0, kbeg), LD, 16 * i + kbeg, &S_(kbeg, c0), 1, LD, 0, 0,
-                             4 * (i + 1), acc, lane);
+            acc = mfma_block<kLeaf / kB - 1>(&S_(0, kbeg), LD, 16 * i + kbeg, &S_(kbeg, c0), 1,
+                                             LD, 0, 0, i + 1, acc, lane);
 #pragma unroll
             for (int e = 0; e < 4; ++e)
                 T[(lane & 15) * (kLeaf - kB) + 16 * i + (lane >> 4) + 4 * e] = acc[e];
@@ -255,7 +279,7 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
         // column block jb of inv(L): rows below = -T * D; diagonal block = D
         for (int i = wave; i < nti; i += NTL / 64) {
             f64x4 acc = {0, 0, 0, 0};
-            acc = mfma_block(T, kLeaf - kB, 16 * i, D, kB, 1, 0, 0, kB / 4, acc, lane);
+            acc = mfma_block<1>(T, kLeaf - kB, 16 * i, D, kB, 1, 0, 0, 1, acc, lane);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int rr = 16 * i + (lane >> 4) + 4 * e;
