@@ -13,7 +13,9 @@
 //   factor: [wave 0] 16x16 diagonal block in registers (readlane broadcasts, 1/sqrt of each
 //           pivot by a refined hardware estimate), then D = inv(L_bb) by right-looking
 //           substitution, kept in the unused block above the diagonal; [all] rows below as
-//           A * D^T and the rank-16 update of the trailing lower triangle, both on f64 MFMA;
+//           A * D^T on f64 MFMA; the rank-16 update of the trailing lower triangle (f64 MFMA)
+//           runs one block late, beside the next diagonal factor: wave 0 updates that
+//           diagonal block first, waves 1-3 the rest;
 //   invert: block columns from last to first: [all] T = inv(L)[below, below] * L[below, j]
 //           and column j of inv(L) = -T * D, both on f64 MFMA, 16-row tiles per wave;
 //   store:  L and inv(L) with 16 LDS reads per lane in flight before the global stores.
@@ -125,8 +127,35 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
     __syncthreads();
 
     // ------------------------------------------------------------------ factorization
+    // Trailing update of block kp's trailing lower triangle, tile t (16 x 16, ti >= tj, tiles
+    // numbered row by row), K = 16 on MFMA: S -= P * P^T with P[r][k] = S_(r, kp + k).
+    auto trail_tile = [&](int kp, int t) {
+        const int base = kp + kB;
+        int ti = 0, acc0 = 0;
+        while (acc0 + ti + 1 <= t) acc0 += ++ti;
+        const int tj = t - acc0;
+        f64x4 acc = {0, 0, 0, 0};
+        // B operand is P^T: element (k, c) = S_(c, kp + k)
+        acc = mfma_block<1>(&S_(0, kp), LD, base + 16 * ti, &S_(0, kp), LD, 1, base + 16 * tj, 0,
+                            1, acc, lane);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = base + 16 * ti + (lane >> 4) + 4 * e;
+            const int c = base + 16 * tj + (lane & 15);
+            if (r < nb && c < nb && r >= c) S_(r, c) -= acc[e];
+        }
+    };
     for (int kb = 0; kb < nb; kb += kB) {
         const int bs = min(kB, nb - kb);
+        // Look-ahead: the previous block's trailing update runs here, beside this block's
+        // diagonal factor.  Wave 0 takes tile 0 (this block's diagonal block) and then factors
+        // it; waves 1-3 take the other tiles, which this factor neither reads nor writes.
+        if (kb > 0) {
+            const int ht = (nb - kb + 15) / 16, ntile = ht * (ht + 1) / 2;
+            if (wave == 0) trail_tile(kb - kB, 0);
+            else
+                for (int t = wave; t < ntile; t += NTL / 64 - 1) trail_tile(kb - kB, t);
+        }
         if (wave == 0) {
             // Diagonal block: lane i holds row i in registers; column j is broadcast with
             // readlane (uniform lane index), so the block needs no LDS round trips.
@@ -204,29 +233,6 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
                 for (int e = 0; e < 4; ++e) {
                     const int r = r0 + (lane >> 4) + 4 * e;
                     if (r < nb) S_(r, kb + (lane & 15)) = acc[e];
-                }
-            }
-        }
-        __syncthreads();
-        // trailing update of the lower triangle on MFMA: 16 x 16 tiles (ti >= tj) of the
-        // trailing part, K = 16 (bs is 16 whenever rows remain below the block)
-        if (rest > 0) {
-            const int base = kb + kB;
-            const int ht = (rest + 15) / 16;
-            const int ntile = ht * (ht + 1) / 2;
-            for (int t = wave; t < ntile; t += NTL / 64) {
-                int ti = 0, acc0 = 0;
-                while (acc0 + ti + 1 <= t) acc0 += ++ti;
-                const int tj = t - acc0;
-                f64x4 acc = {0, 0, 0, 0};
-                // P[r][k] = S_(r, kb + k); B operand is P^T: element (k, c) = S_(c, kb + k)
-                acc = mfma_block<1>(&S_(0, kb), LD, base + 16 * ti, &S_(0, kb), LD, 1,
-                                    base + 16 * tj, 0, 1, acc, lane);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int r = base + 16 * ti + (lane >> 4) + 4 * e;
-                    const int c = base + 16 * tj + (lane & 15);
-                    if (r < nb && c < nb && r >= c) S_(r, c) -= acc[e];
                 }
             }
         }
