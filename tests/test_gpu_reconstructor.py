@@ -133,3 +133,21 @@ def test_not_positive_definite_reports_info():
     info = reconstructor(n, m, C.copy(order="F"), n, Bg, m, 1)
     assert info == info_ref == 301
     assert np.array_equal(Bg, B)  # R not computed: Ctm untouched on the host entry
+
+
+def test_nan_pivot_reports_info():
+    """A NaN that reaches a pivot stops the factorization as a non-positive pivot does (LAPACK's
+    dpotrf tests each pivot with `ajj <= 0 or disnan(ajj)`).  A NaN at (200, 150) leaves the
+    pivots of columns 0..199 finite and reaches the pivot of column 200 first: info = 201, from
+    inside the second 128-column leaf.  The oracle is not used here: OpenBLAS's own potrf does
+    not test for NaN."""
+    print("[recon] NaN pivot info", flush=True)
+    rng = np.random.default_rng(11)
+    n, m = 400, 10
+    G = rng.standard_normal((n, n))
+    C = np.asfortranarray(np.tril(G @ G.T / n + np.eye(n)))
+    C[200, 150] = np.nan
+    B = np.asfortranarray(rng.standard_normal((m, n)))
+    Bg = B.copy(order="F")
+    assert reconstructor(n, m, C.copy(order="F"), n, Bg, m, 1) == 201
+    assert np.array_equal(Bg, B)
