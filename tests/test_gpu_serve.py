@@ -6,14 +6,17 @@ Bars:
   * FP64 requests match the oracle's LAPACK path on the same Cmm/Ctm within 1e-10 relative
     Frobenius error (the FP64 parity bar); mixed requests within 1e-6 of it;
   * admission never lets the requests in flight hold more than the byte budget, and a request
-    that alone exceeds the budget is refused at submit with MOAO_ERR_NOMEM.
+    that alone exceeds the budget is refused at submit with MOAO_ERR_NOMEM;
+  * a request that fails on the device (a mixed solve that cannot converge) reports its status
+    and leaves the requests beside it, and the ones after it, exactly as they are alone.
 """
 import numpy as np
 import pytest
 
 from moao_amd import matcov as gm
 from moao_amd import systems
-from moao_amd.lib import MOAO_ERR_NOMEM, MOAO_FP64, MOAO_MIXED_FP32, MoaoError
+from moao_amd.lib import (MOAO_ERR_NOCONV, MOAO_ERR_NOMEM, MOAO_FP64, MOAO_MIXED_FP32,
+                          MoaoError)
 from moao_amd.serve import Server
 from moao_amd.tomo import Tomo
 from oracle import lapack_path as lp
@@ -96,3 +99,32 @@ def test_admission_respects_budget(tmp_path):
         with pytest.raises(MoaoError) as e:
             srv.submit(s, a, 0, MOAO_FP64, want_R=False)
         assert e.value.code == MOAO_ERR_NOMEM
+
+
+def test_failed_request_is_isolated(tmp_path):
+    syst = _systems(tmp_path)[:4]
+    print("[serve] failing request in a mix", flush=True)
+    with Server(slots=3) as srv:
+        tickets = []
+        for k, (s, a, n, m) in enumerate(syst):
+            if k == 1:  # cannot reach the tolerance in one refinement step
+                tickets.append(srv.submit(s, a, 0, MOAO_MIXED_FP32, tol=1e-30, max_iter=1,
+                                          n_m=(n, m)))
+            else:
+                tickets.append(srv.submit(s, a, 0, MOAO_FP64, n_m=(n, m)))
+        res = [srv.wait(t, raise_on_error=False) for t in tickets]
+        s, a, n, m = syst[1]  # the server keeps serving after the failure
+        after = srv.wait(srv.submit(s, a, 0, MOAO_FP64, n_m=(n, m)), raise_on_error=False)
+        st = srv.stats()
+    assert res[1]["status"] == MOAO_ERR_NOCONV and res[1]["R"] is None
+    assert after["status"] == 0
+    assert st["peak_running"] >= 2
+    with Server(slots=1) as srv:
+        alone = {k: srv.wait(srv.submit(s, a, 0, MOAO_FP64, n_m=(n, m)))
+                 for k, (s, a, n, m) in enumerate(syst) if k != 1}
+    for k, r in alone.items():
+        assert res[k]["status"] == 0
+        assert np.array_equal(res[k]["R"], r["R"]), f"request {k} differs beside a failure"
+    s, a, n, m = syst[1]
+    with Server(slots=1) as srv:
+        assert np.array_equal(after["R"], srv.wait(srv.submit(s, a, 0, MOAO_FP64, n_m=(n, m)))["R"])
