@@ -199,25 +199,9 @@ def serving(nreq, mode, slots, seed=2024, util=0.8):
     from moao_amd import systems
     from moao_amd.lib import MOAO_FP64, MOAO_MIXED_FP32
     from moao_amd.serve import Server
-    rng = np.random.default_rng(seed)
+    rng = np.random.default_rng(seed + 1)
     tmp = tempfile.mkdtemp(prefix="moao_serve_")
-    reqs = []
-    while len(reqs) < nreq:
-        nw = int(rng.integers(4, 10))
-        nssp = int(rng.integers(20, 75))
-        nv = systems.valid_subaps(nssp, 0.28)
-        n = 2 * nw * nv  # n_wfs sensors measure; the truth sensor is extra
-        if n > 40000:
-            continue
-        nlgs = max(1, (2 * nw + 2) // 3)
-        spec = systems.SystemSpec(diam=39.0, nssp=nssp, nlgs=nlgs, nngs=nw - nlgs, nlayer=10,
-                                  r0=float(rng.uniform(0.08, 0.2)),
-                                  seed=int(rng.integers(1 << 30)))
-        d = os.path.join(tmp, f"r{len(reqs)}")
-        os.makedirs(d)
-        s, atms = systems.make_system(d, spec)
-        m = 2 * nv
-        reqs.append((s, atms[0], n, m))
+    reqs = systems.request_stream(tmp, nreq, seed)
     # service-time model of one request on an otherwise idle GPU: solve at 40 TFLOP/s,
     # matcov at 2.3e9 elements/s (10 layers), R to pageable host memory at 20 GB/s
     est = [solve_flops(n, m) / 40e12 + (n * (n + 1) / 2 + m * n) / 2.3e9 + 8.0 * m * n / 20e9

@@ -102,6 +102,11 @@ int moao_hip_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int moao_hip_synchronize(void);
 /* Device memory in use / total, bytes. */
 int moao_hip_mem_info(size_t* free_bytes, size_t* total_bytes);
+/* Streams for the stream-ordered entries (default priority; NULL means the library stream).
+ * No reference counterpart: the reference's CPU calls are synchronous. */
+int moao_hip_stream_create(void** stream);
+int moao_hip_stream_synchronize(void* stream);
+int moao_hip_stream_destroy(void* stream);
 
 /* ---------------------------------------------------------------- matcov ------------------
  * MOAO_COMMON::matcov_tile (common/compute/matcov.hpp:14-18, matcov.cpp:17-98).
@@ -122,9 +127,17 @@ int moao_hip_matcov_tile_dev(double* d_data, int nrows, int ncols, int xoffset, 
  * left partially factored and, for the host entry, Ctm untouched), or MOAO_ERR_*.            */
 int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, double* Ctm,
                            long ldCtm, long ngal);
-/* Device buffers; on info > 0 the Ctm blocks hold partial results. */
+/* Device buffers; returns when the solve is done (its status is read back).  On info > 0 the
+ * Ctm blocks hold partial results.  Workspace comes from the stream-ordered pool, per call, so
+ * calls on different streams may overlap. */
 int moao_hip_reconstructor_dev(long nmeas, long nmeasts, double* dCmm, long ldCmm, double* dCtm,
                                long ldCtm, long ngal, void* stream);
+/* The same solve fully stream-ordered: returns once the work is queued on `stream`; the dpotrf
+ * status is written to the device int *d_info when the solve runs (0, or the order of the first
+ * non-positive pivot, after which every later kernel of the solve does nothing).  No
+ * counterpart in the reference (its calls are synchronous); for callers that pipeline solves. */
+int moao_hip_reconstructor_async(long nmeas, long nmeasts, double* dCmm, long ldCmm,
+                                 double* dCtm, long ldCtm, long ngal, int* d_info, void* stream);
 
 /* Mixed precision (no reference counterpart; same result contract as the FP64 entry to a
  * stated tolerance).  Cmm is read only (lower triangle) and preserved; the Ctm block (one
@@ -217,7 +230,8 @@ typedef struct moao_serve_request {
     double tol;            /* mixed: relative residual to refine to (0 -> 1e-11) */
     int max_iter;          /* mixed: refinement steps allowed (0 -> 5) */
     double* R;             /* host output m x n (column-major) or NULL; written before wait */
-    long ldR;
+    long ldR;              /* leading dimension of R (>= m) */
+    long R_cols;           /* columns R holds; must equal n (checked at submit) */
 } moao_serve_request;
 
 typedef struct moao_serve_result {

@@ -42,7 +42,6 @@ struct Context {
     DevBuf host_stage;      // device copy of a host matrix for the host-pointer entry points
     DevBuf host_stage2;
     DevBuf recon_ws;        // reconstructor workspace (inverse leaves + status word)
-    DevBuf mixed_ws;        // mixed-precision reconstructor workspace
     std::mutex mu;
 };
 

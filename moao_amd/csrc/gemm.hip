@@ -72,6 +72,7 @@ __global__ void __launch_bounds__(NT, WMB == 4 ? 2 : 4) gemm_kernel(GemmArgs<T> 
     __shared__ T As[2][BK][TM + PAD];
     __shared__ T Bs[2][BK][BN + PAD];
 
+    if (g.skip_if && *(volatile const int*)g.skip_if != 0) return;
     const GemmSeg<T> s = g.seg[blockIdx.z];
     const int m0 = blockIdx.x * TM;
     const int n0 = blockIdx.y * BN;
@@ -274,6 +275,7 @@ __global__ void __launch_bounds__(NT, TM == 128 ? 2 : 4) gemm_dl_kernel(GemmArgs
     __shared__ T As[S][BK][LDA];
     __shared__ T Bs[S][BSZ];
 
+    if (g.skip_if && *(volatile const int*)g.skip_if != 0) return;
     const GemmSeg<T> s = g.seg[blockIdx.z];
     const int m0 = blockIdx.x * TM;
     const int n0 = blockIdx.y * BN;

@@ -44,6 +44,10 @@ struct GemmArgs {
     // transb = 2 only, optional: full symmetric copies of B's 128 x 128 diagonal blocks
     // (gemm_symdiag); with it the launch runs on the direct-to-LDS kernel.
     const T* symdiag;
+    // Optional device status word: when it is non-zero at launch time the whole launch is a
+    // no-op.  The factorization points it at its dpotrf info, so the updates and solves queued
+    // after a failing pivot do nothing and no host synchronisation is needed to stop them.
+    const int* skip_if;
 };
 
 // Elements of the gemm_symdiag copy of a K x K matrix.
@@ -77,6 +81,7 @@ inline GemmArgs<T> gemm_args(int M, int N, int K, T alpha, const T* A, long lda,
     g.alpha = alpha;
     g.beta = beta;
     g.symdiag = nullptr;
+    g.skip_if = nullptr;
     return g;
 }
 

@@ -4,7 +4,8 @@
 // LAPACK semantics of dpotrf('L') for the block: the lower triangle is overwritten by L, the
 // strictly-upper part is not touched; a non-positive (or NaN) pivot at local column j stops
 // the factorization and records info = j_global + 1 (first failure wins; later leaves see the
-// flag and return at once).  The inverse L^{-1} (lower, zero above the diagonal) goes to a
+// flag and return at once); the leaf's 16-column blocks before the failing pivot's block are
+// stored, the rest of the block keeps its input.  The inverse L^{-1} (lower, zero above the diagonal) goes to a
 // separate kLeaf x kLeaf buffer and turns every panel triangular solve into an MFMA GEMM.
 //
 // One workgroup (4 waves) per leaf; the block is zero-padded to kLeaf x kLeaf in LDS and
@@ -126,6 +127,27 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
     }
     __syncthreads();
 
+    // Columns [0, ncol) of L back to global (lower part only), 16 LDS reads per lane in flight
+    // before the stores.
+    auto store_L = [&](int ncol) {
+        for (int cb = wave; cb < ncol; cb += 4 * 8) {
+            double v0[8], v1[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int c = min(cb + 4 * q, kLeaf - 1);
+                v0[q] = S_(lane, c);
+                v1[q] = S_(lane + 64, c);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int c = cb + 4 * q;
+                if (c >= ncol) break;
+                if (lane >= c && lane < nb) A[lane + (long)c * lda] = (TS)v0[q];
+                if (lane + 64 >= c && lane + 64 < nb) A[lane + 64 + (long)c * lda] = (TS)v1[q];
+            }
+        }
+    };
+
     // ------------------------------------------------------------------ factorization
     // Trailing update of block kp's trailing lower triangle, tile t (16 x 16, ti >= tj, tiles
     // numbered row by row), K = 16 on MFMA: S -= P * P^T with P[r][k] = S_(r, kp + k).
@@ -215,7 +237,12 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
             }
         }
         __syncthreads();
-        if (fail) return;
+        if (fail) {
+            // the 16-column blocks before the failing one are final: store them (their rows
+            // below included), leave the failing block and the rest as they were
+            store_L(kb);
+            return;
+        }
         // rows below the diagonal block: L[r, kb:kb+16] = A[r, kb:kb+16] * D^T on MFMA, a
         // 16-row tile per wave turn, in place (bs is 16 whenever rows remain below the block)
         const int rest = nb - kb - bs;
@@ -238,23 +265,7 @@ __global__ void __launch_bounds__(NTL) leaf_potrf_inv(TS* A, long lda, int nb, T
         }
         __syncthreads();
     }
-    // L back to global (lower part only), 16 LDS reads per lane in flight before the stores.
-    for (int cb = wave; cb < nb; cb += 4 * 8) {
-        double v0[8], v1[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int c = min(cb + 4 * q, kLeaf - 1);
-            v0[q] = S_(lane, c);
-            v1[q] = S_(lane + 64, c);
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int c = cb + 4 * q;
-            if (c >= nb) break;
-            if (lane >= c && lane < nb) A[lane + (long)c * lda] = (TS)v0[q];
-            if (lane + 64 >= c && lane + 64 < nb) A[lane + 64 + (long)c * lda] = (TS)v1[q];
-        }
-    }
+    store_L(nb);
 
     // ------------------------------------------------------------------ inverse
     const int nblocks = (nb + kB - 1) / kB;

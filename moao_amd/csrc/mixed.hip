@@ -207,7 +207,8 @@ size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank) {
 
 int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, double* Ctm, long ldCtm,
                             const moao_tomo_view* lowrank, double tol, int max_iter,
-                            double* resid_out, int* iters_out, DevBuf& wsbuf, hipStream_t s) {
+                            double* resid_out, int* iters_out, void* ws, size_t ws_bytes,
+                            hipStream_t s) {
     if (n < 1 || m < 1 || ldCmm < n || ldCtm < m || !Cmm || !Ctm || max_iter < 0 ||
         n >= (1L << 31) || m >= (1L << 31)) {
         set_error("reconstructor_mixed: invalid arguments");
@@ -219,8 +220,11 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
     const long xr = m + p;  // rows of the FP32 right-hand-side block
     std::vector<size_t> offs;
     const size_t total = mixed_layout(n, m, p, &offs);
-    if (int rc = wsbuf.ensure(total)) return rc;
-    char* b = wsbuf.as<char>();
+    if (!ws || ws_bytes < total) {
+        set_error("reconstructor_mixed: workspace too small");
+        return MOAO_ERR_ARG;
+    }
+    char* b = static_cast<char*>(ws);
     MixedWs w{(float*)(b + offs[0]), (float*)(b + offs[1]), (float*)(b + offs[2]),
               (double*)(b + offs[3]), (double*)(b + offs[4]), (double*)(b + offs[5]),
               (double*)(b + offs[6]), (double*)(b + offs[7]), (double*)(b + offs[8]),
@@ -395,10 +399,28 @@ int moao_hip_reconstructor_mixed_dev(long nmeas, long nmeasts, const double* dCm
         moao::set_error("reconstructor_mixed: unsupported mode");
         return MOAO_ERR_ARG;
     }
-    std::lock_guard<std::mutex> g(moao::ctx().mu);
-    return moao::reconstructor_mixed_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, lowrank, tol,
-                                         max_iter, resid, iters, moao::ctx().mixed_ws,
-                                         moao::pick_stream(stream));
+    if (nmeas < 1 || nmeasts < 1 || nmeas >= (1L << 31) || nmeasts >= (1L << 31)) {
+        moao::set_error("reconstructor_mixed: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    // workspace of this call only, from the stream-ordered pool
+    hipStream_t s = moao::pick_stream(stream);
+    const size_t bytes = moao::mixed_workspace_bytes(nmeas, nmeasts, lowrank);
+    void* ws = nullptr;
+    if (hipMallocAsync(&ws, bytes, s) != hipSuccess) {
+        (void)hipGetLastError();
+        moao::set_error("reconstructor_mixed: workspace allocation of " + std::to_string(bytes) +
+                        " B failed");
+        return MOAO_ERR_NOMEM;
+    }
+    int rc = moao::reconstructor_mixed_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, lowrank, tol,
+                                           max_iter, resid, iters, ws, bytes, s);
+    (void)hipFreeAsync(ws, s);
+    if (hipStreamSynchronize(s) != hipSuccess && rc == MOAO_OK) {
+        moao::set_error("reconstructor_mixed: stream synchronisation failed");
+        rc = MOAO_ERR_HIP;
+    }
+    return rc;
 }
 
 }  // extern "C"

@@ -44,6 +44,7 @@ struct Tall {
     const std::vector<RowGroup<T>>* groups;
     T* invL;
     int* info;
+    const int* skip;  // device status word: launches are no-ops once it is non-zero
     hipStream_t s;
     T* a(long r, long c) const { return A + r + c * lda; }
     T* inv(long j0) const { return invL + (j0 / kLeaf) * (long)kLeaf * kLeaf; }
@@ -87,6 +88,7 @@ int potrf_rec(const Tall<T>& p, long j0, long nn) {
         // Panel: rows below the leaf and every group row, times inv(L)^T, in place.
         const long below = p.n - j0 - nn;
         GemmArgs<T> proto{};
+        proto.skip_if = p.skip;
         proto.B = p.inv(j0);
         proto.ldb = kLeaf;
         proto.N = (int)nn;
@@ -108,6 +110,7 @@ int potrf_rec(const Tall<T>& p, long j0, long nn) {
     // Columns [j0+n1, j0+nn) of every row below j0+n1 -= rows * A[j0+n1:j0+nn, j0:j0+n1]^T
     const long r0 = j0 + n1;
     GemmArgs<T> proto{};
+    proto.skip_if = p.skip;
     proto.B = p.a(r0, j0);
     proto.ldb = p.lda;
     proto.N = (int)(nn - n1);
@@ -135,6 +138,7 @@ int trsmN_rec(const Tall<T>& p, long j0, long nn) {
     auto none = [] { return GemmSeg<T>{}; };
     if (nn <= kLeaf) {
         GemmArgs<T> proto{};
+        proto.skip_if = p.skip;
         proto.B = p.inv(j0);
         proto.ldb = kLeaf;
         proto.N = (int)nn;
@@ -153,6 +157,7 @@ int trsmN_rec(const Tall<T>& p, long j0, long nn) {
     if (int rc = trsmN_rec(p, j0 + n1, nn - n1)) return rc;
     // X[:, j0:j0+n1] -= Y[:, j0+n1:j0+nn] * L[j0+n1:j0+nn, j0:j0+n1]
     GemmArgs<T> proto{};
+    proto.skip_if = p.skip;
     proto.B = p.a(j0 + n1, j0);
     proto.ldb = p.lda;
     proto.N = (int)n1;
@@ -177,6 +182,7 @@ int trsmT_rec(const Tall<T>& p, long j0, long nn) {
     auto none = [] { return GemmSeg<T>{}; };
     if (nn <= kLeaf) {
         GemmArgs<T> proto{};
+        proto.skip_if = p.skip;
         proto.B = p.inv(j0);
         proto.ldb = kLeaf;
         proto.N = (int)nn;
@@ -195,6 +201,7 @@ int trsmT_rec(const Tall<T>& p, long j0, long nn) {
     if (int rc = trsmT_rec(p, j0, n1)) return rc;
     // B[:, j0+n1:j0+nn] -= X[:, j0:j0+n1] * L[j0+n1:j0+nn, j0:j0+n1]^T
     GemmArgs<T> proto{};
+    proto.skip_if = p.skip;
     proto.B = p.a(j0 + n1, j0);
     proto.ldb = p.lda;
     proto.N = (int)(nn - n1);
@@ -214,25 +221,31 @@ int trsmT_rec(const Tall<T>& p, long j0, long nn) {
 
 }  // namespace
 
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
 template <typename T>
 int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
                int* info, hipStream_t s) {
-    Tall<T> p{n, A, lda, &groups, invL, info, s};
-    MOAO_HIP_TRY(hipMemsetAsync(info, 0, sizeof(int), s));
+    Tall<T> p{n, A, lda, &groups, invL, info, info, s};
+    // a kernel rather than a memset node, so the sequence can be captured into a graph
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, info, 0);
+    MOAO_HIP_TRY(hipGetLastError());
     return potrf_rec(p, 0, n);
 }
 
 template <typename T>
 int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
-                             const std::vector<RowGroup<T>>& groups, hipStream_t s) {
-    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, s};
+                             const std::vector<RowGroup<T>>& groups, hipStream_t s,
+                             const int* skip) {
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s};
     return trsmN_rec(p, 0, n);
 }
 
 template <typename T>
 int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
-                           const std::vector<RowGroup<T>>& groups, hipStream_t s) {
-    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, s};
+                           const std::vector<RowGroup<T>>& groups, hipStream_t s,
+                           const int* skip) {
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s};
     return trsmT_rec(p, 0, n);
 }
 
@@ -241,62 +254,102 @@ template int potrf_tall<double>(long, double*, long, const std::vector<RowGroup<
 template int potrf_tall<float>(long, float*, long, const std::vector<RowGroup<float>>&, float*,
                                int*, hipStream_t);
 template int trsm_right_lower_notrans<double>(long, const double*, long, const double*,
-                                              const std::vector<RowGroup<double>>&, hipStream_t);
+                                              const std::vector<RowGroup<double>>&, hipStream_t,
+                                              const int*);
 template int trsm_right_lower_notrans<float>(long, const float*, long, const float*,
-                                             const std::vector<RowGroup<float>>&, hipStream_t);
+                                             const std::vector<RowGroup<float>>&, hipStream_t,
+                                             const int*);
 template int trsm_right_lower_trans<double>(long, const double*, long, const double*,
-                                            const std::vector<RowGroup<double>>&, hipStream_t);
+                                            const std::vector<RowGroup<double>>&, hipStream_t,
+                                            const int*);
 template int trsm_right_lower_trans<float>(long, const float*, long, const float*,
-                                           const std::vector<RowGroup<float>>&, hipStream_t);
+                                           const std::vector<RowGroup<float>>&, hipStream_t,
+                                           const int*);
 
 int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm, long ldCtm,
                           long ngal, double* dinvL, int* dinfo, hipStream_t stream) {
     if (n < 1 || m < 0 || ngal < 1 || ldCmm < n || (m > 0 && ldCtm < m) || !dCmm || !dinvL ||
-        !dinfo || n >= (1L << 31) || m >= (1L << 31)) {
+        !dinfo || (m > 0 && !dCtm) || n >= (1L << 31) || m >= (1L << 31)) {
         set_error("reconstructor: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    // Target g's block starts at g*n*m (lapack/compute/reconstructor.cpp:43); with ldCtm > m
+    // consecutive blocks would overlap and be updated concurrently.
+    if (ngal > 1 && m > 0 && ldCtm != m) {
+        set_error("reconstructor: several targets need ldCtm == nmeasts (blocks at g*n*m)");
         return MOAO_ERR_ARG;
     }
     std::vector<RowGroup<double>> groups;
     if (m > 0)
         for (long g = 0; g < ngal; ++g) groups.push_back({dCtm + g * n * m, ldCtm, (int)m});
     if (int rc = potrf_tall<double>(n, dCmm, ldCmm, groups, dinvL, dinfo, stream)) return rc;
-    int hinfo = 0;
-    MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, stream));
-    MOAO_HIP_TRY(hipStreamSynchronize(stream));
-    if (hinfo != 0) return hinfo;
+    // the second solve is predicated on the status word: after a failing pivot it does nothing
     if (m > 0)
-        if (int rc = trsm_right_lower_notrans<double>(n, dCmm, ldCmm, dinvL, groups, stream))
+        if (int rc = trsm_right_lower_notrans<double>(n, dCmm, ldCmm, dinvL, groups, stream, dinfo))
             return rc;
     return MOAO_OK;
 }
 
-}  // namespace moao
-
-namespace {
-
-// Device workspace of one FP64 solve: the inverse leaves, then the status word.
-int recon_workspace(long n, double** invL, int** info) {
-    moao::Context& c = moao::ctx();
-    const size_t bytes = moao::recon_invl_doubles(n) * sizeof(double) + 256;
-    if (int rc = c.recon_ws.ensure(bytes)) return rc;
-    *invL = c.recon_ws.as<double>();
-    *info = reinterpret_cast<int*>(c.recon_ws.as<char>() + bytes - 256);
-    return MOAO_OK;
+size_t recon_workspace_bytes(long n) {
+    return ((recon_invl_doubles(n) * sizeof(double) + 255) & ~size_t(255)) + 256;
 }
 
-}  // namespace
+}  // namespace moao
 
 extern "C" {
+
+int moao_hip_reconstructor_async(long nmeas, long nmeasts, double* dCmm, long ldCmm,
+                                 double* dCtm, long ldCtm, long ngal, int* d_info, void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (!d_info || nmeas < 1) {
+        moao::set_error("reconstructor_async: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    hipStream_t s = moao::pick_stream(stream);
+    // workspace of this call only, from the stream-ordered pool: calls on different streams
+    // never share it
+    char* ws = nullptr;
+    const size_t bytes = moao::recon_workspace_bytes(nmeas);
+    if (hipMallocAsync(reinterpret_cast<void**>(&ws), bytes, s) != hipSuccess) {
+        (void)hipGetLastError();
+        moao::set_error("reconstructor_async: workspace allocation failed");
+        return MOAO_ERR_NOMEM;
+    }
+    double* invL = reinterpret_cast<double*>(ws);
+    int* info = reinterpret_cast<int*>(ws + bytes - 256);
+    int rc = moao::reconstructor_f64_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, ngal, invL,
+                                         info, s);
+    if (rc == MOAO_OK) {
+        if (hipMemcpyAsync(d_info, info, sizeof(int), hipMemcpyDeviceToDevice, s) != hipSuccess) {
+            moao::set_error("reconstructor_async: status copy failed");
+            rc = MOAO_ERR_HIP;
+        }
+    }
+    (void)hipFreeAsync(ws, s);
+    return rc;
+}
 
 int moao_hip_reconstructor_dev(long nmeas, long nmeasts, double* dCmm, long ldCmm, double* dCtm,
                                long ldCtm, long ngal, void* stream) {
     if (int rc = moao::ensure_init()) return rc;
-    double* invL;
-    int* info;
-    std::lock_guard<std::mutex> g(moao::ctx().mu);
-    if (int rc = recon_workspace(nmeas, &invL, &info)) return rc;
-    return moao::reconstructor_f64_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, ngal, invL,
-                                       info, moao::pick_stream(stream));
+    hipStream_t s = moao::pick_stream(stream);
+    int* dinfo = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&dinfo), sizeof(int), s) != hipSuccess) {
+        (void)hipGetLastError();
+        moao::set_error("reconstructor: status allocation failed");
+        return MOAO_ERR_NOMEM;
+    }
+    int rc = moao_hip_reconstructor_async(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, ngal, dinfo,
+                                          s);
+    int hinfo = 0;
+    if (rc == MOAO_OK &&
+        hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) {
+        moao::set_error("reconstructor: status copy failed");
+        rc = MOAO_ERR_HIP;
+    }
+    (void)hipFreeAsync(dinfo, s);
+    MOAO_HIP_TRY(hipStreamSynchronize(s));
+    return rc != MOAO_OK ? rc : hinfo;
 }
 
 int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, double* Ctm,
@@ -310,29 +363,35 @@ int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, do
     moao::Context& c = moao::ctx();
     hipStream_t s = c.stream;
     std::lock_guard<std::mutex> g(c.mu);
-    const size_t a_bytes = (size_t)ldCmm * nmeas * sizeof(double);
+    // LAPACK's minimum extent of an n x n matrix with leading dimension ldCmm
+    const size_t a_bytes = ((size_t)(nmeas - 1) * ldCmm + nmeas) * sizeof(double);
     // Ctm of target g starts at g*nmeas*nmeasts (lapack/compute/reconstructor.cpp:43).
     const size_t b_elems = nmeasts > 0 ? (size_t)(ngal - 1) * nmeas * nmeasts +
                                              (size_t)(nmeas - 1) * ldCtm + nmeasts
                                        : 0;
+    const size_t ws_bytes = moao::recon_workspace_bytes(nmeas);
     if (int rc = c.host_stage.ensure(a_bytes)) return rc;
     if (int rc = c.host_stage2.ensure(b_elems * sizeof(double) + 8)) return rc;
+    if (int rc = c.recon_ws.ensure(ws_bytes)) return rc;
     double* dA = c.host_stage.as<double>();
     double* dB = c.host_stage2.as<double>();
+    double* invL = c.recon_ws.as<double>();
+    int* info = reinterpret_cast<int*>(c.recon_ws.as<char>() + ws_bytes - 256);
     MOAO_HIP_TRY(hipMemcpyAsync(dA, Cmm, a_bytes, hipMemcpyHostToDevice, s));
     if (b_elems)
         MOAO_HIP_TRY(hipMemcpyAsync(dB, Ctm, b_elems * sizeof(double), hipMemcpyHostToDevice, s));
-    double* invL;
-    int* info;
-    if (int rc = recon_workspace(nmeas, &invL, &info)) return rc;
     int rc = moao::reconstructor_f64_dev(nmeas, nmeasts, dA, ldCmm, dB, ldCtm, ngal, invL, info, s);
     if (rc < 0) return rc;
-    // dpotrf overwrites Cmm with its factor even when it stops early; R only on success.
+    int hinfo = 0;
+    MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
+    MOAO_HIP_TRY(hipStreamSynchronize(s));
+    // Cmm gets the factor (on a failing pivot: the columns of the leaves before it, and the
+    // 16-column blocks before the failing one within its leaf); R only on success.
     MOAO_HIP_TRY(hipMemcpyAsync(Cmm, dA, a_bytes, hipMemcpyDeviceToHost, s));
-    if (rc == 0 && b_elems)
+    if (hinfo == 0 && b_elems)
         MOAO_HIP_TRY(hipMemcpyAsync(Ctm, dB, b_elems * sizeof(double), hipMemcpyDeviceToHost, s));
     MOAO_HIP_TRY(hipStreamSynchronize(s));
-    return rc;
+    return hinfo;
 }
 
 }  // extern "C"

@@ -25,22 +25,29 @@ template <typename T>
 int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
                int* info, hipStream_t s);
 // Y * L = X for every group (second right-side solve), with L / invL from potrf_tall.
+// skip: optional device status word; every launch is a no-op once it is non-zero.
 template <typename T>
 int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
-                             const std::vector<RowGroup<T>>& groups, hipStream_t s);
+                             const std::vector<RowGroup<T>>& groups, hipStream_t s,
+                             const int* skip = nullptr);
 // X * L^T = B for every group (first right-side solve, standalone).
 template <typename T>
 int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
-                           const std::vector<RowGroup<T>>& groups, hipStream_t s);
+                           const std::vector<RowGroup<T>>& groups, hipStream_t s,
+                           const int* skip = nullptr);
 
-// FP64 path.  dCmm (n x n, lower) is overwritten by its Cholesky factor L; each of the ngal
-// target blocks Ctm_g = dCtm + g*n*m (m x n, leading dimension ldCtm) is overwritten by R_g.
-// On a non-positive pivot the return value is its 1-based order and the Ctm blocks hold
-// partial results.  dinvL: workspace of recon_invl_elems(n) doubles; dinfo: one device int.
+// FP64 path, stream-ordered (no host synchronisation).  dCmm (n x n, lower) is overwritten by
+// its Cholesky factor L; each of the ngal target blocks Ctm_g = dCtm + g*n*m (m x n, leading
+// dimension ldCtm) is overwritten by R_g.  *dinfo (device) receives the dpotrf status: 0, or
+// the 1-based order of the first non-positive pivot, after which every later launch of the
+// solve is a no-op (the Ctm blocks then hold partial results).  dinvL: workspace of
+// recon_invl_doubles(n) doubles.
 int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm, long ldCtm,
                           long ngal, double* dinvL, int* dinfo, hipStream_t stream);
 
 // Number of elements of the inverse-leaf workspace.
 size_t recon_invl_doubles(long n);
+// Bytes of one FP64 solve's workspace: the inverse leaves, then (last 256 bytes) the status.
+size_t recon_workspace_bytes(long n);
 
 }  // namespace moao

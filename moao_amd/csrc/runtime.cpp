@@ -229,7 +229,6 @@ int moao_hip_finalize(void) {
     c.host_stage.release();
     c.host_stage2.release();
     c.recon_ws.release();
-    c.mixed_ws.release();
     return MOAO_OK;
 }
 
@@ -280,6 +279,28 @@ int moao_hip_synchronize(void) {
 int moao_hip_mem_info(size_t* free_bytes, size_t* total_bytes) {
     if (int rc = moao::ensure_init()) return rc;
     MOAO_HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+    return MOAO_OK;
+}
+
+int moao_hip_stream_create(void** out) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (!out) return MOAO_ERR_ARG;
+    hipStream_t s = nullptr;
+    // default priority, like every stream of the library
+    MOAO_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = s;
+    return MOAO_OK;
+}
+
+int moao_hip_stream_synchronize(void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    MOAO_HIP_TRY(hipStreamSynchronize(moao::pick_stream(stream)));
+    return MOAO_OK;
+}
+
+int moao_hip_stream_destroy(void* stream) {
+    if (!stream) return MOAO_OK;
+    MOAO_HIP_TRY(hipStreamDestroy(static_cast<hipStream_t>(stream)));
     return MOAO_OK;
 }
 

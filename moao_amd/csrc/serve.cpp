@@ -120,18 +120,30 @@ struct moao_server {
     }
     void worker(int k);
     int run(moao::Request& r, moao::Slot& s);
+    int run_body(moao::Request& r, moao::Slot& s);
 };
 
 int moao_server::run(moao::Request& r, moao::Slot& sl) {
+    const int rc = run_body(r, sl);
+    if (rc && r.error.empty()) r.error = moao_last_error();
+    return rc;
+}
+
+int moao_server::run_body(moao::Request& r, moao::Slot& sl) {
     using namespace moao;
     hipStream_t s = sl.stream;
     const long n = r.n, m = r.m;
     char* block = nullptr;
-    MOAO_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&block), r.footprint, s));
+    if (hipMallocAsync(reinterpret_cast<void**>(&block), r.footprint, s) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("serve: device allocation of " + std::to_string(r.footprint) + " B failed");
+        return MOAO_ERR_NOMEM;
+    }
     double* Cmm = reinterpret_cast<double*>(block);
     double* Ctm = reinterpret_cast<double*>(block + al256(sizeof(double) * n * n));
     char* scratch = block + al256(sizeof(double) * n * n) + al256(sizeof(double) * m * n);
     int rc = MOAO_OK;
+    int hinfo = 0;
     do {
         rc = matcov_tile_launch(Cmm, (int)n, (int)n, 0, 0, (int)n, &r.view, 1, s, &sl.tables);
         if (rc) break;
@@ -139,16 +151,20 @@ int moao_server::run(moao::Request& r, moao::Slot& sl) {
         if (rc) break;
         if (r.mode == MOAO_FP64) {
             double* invL = reinterpret_cast<double*>(scratch);
-            int* info = reinterpret_cast<int*>(scratch + al256(recon_invl_doubles(n) * 8));
+            int* info = reinterpret_cast<int*>(scratch + recon_workspace_bytes(n) - 256);
             rc = reconstructor_f64_dev(n, m, Cmm, n, Ctm, m, 1, invL, info, s);
+            if (rc) break;
+            if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s) !=
+                hipSuccess) {
+                set_error("serve: status copy failed");
+                rc = MOAO_ERR_HIP;
+                break;
+            }
         } else {
-            DevBuf ws;  // wraps the request block: ensure() finds it large enough
-            ws.ptr = scratch;
-            ws.bytes = r.solve_bytes;
             double resid = 0;
             int iters = 0;
             rc = reconstructor_mixed_dev(n, m, Cmm, n, Ctm, m, &r.view, r.tol, r.max_iter, &resid,
-                                         &iters, ws, s);
+                                         &iters, scratch, r.solve_bytes, s);
             r.res.resid = resid;
             r.res.iters = iters;
         }
@@ -167,8 +183,14 @@ int moao_server::run(moao::Request& r, moao::Slot& sl) {
     hipError_t e1 = hipFreeAsync(block, s);
     hipError_t e2 = hipStreamSynchronize(s);
     if (!rc && (e1 != hipSuccess || e2 != hipSuccess)) {
-        r.error = std::string("serve: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2);
+        set_error(std::string("serve: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
         rc = MOAO_ERR_HIP;
+    }
+    if (!rc && hinfo) {
+        set_error("serve: Cmm is not positive definite (dpotrf info " + std::to_string(hinfo) +
+                  ")");
+        r.error = moao_last_error();
+        rc = hinfo;
     }
     return rc;
 }
@@ -264,8 +286,15 @@ int moao_serve_submit(moao_server* s, const moao_serve_request* q, long* ticket)
     r->view = moao::view_of(*r->tomo);
     r->n = r->tomo->nmeas();
     r->m = r->tomo->nmeasts();
-    if (r->n < 1 || r->m < 1 || r->n >= (1L << 31) || (q->R && q->ldR < r->m)) {
-        moao::set_error("serve_submit: invalid system or output leading dimension");
+    if (r->n < 1 || r->m < 1 || r->n >= (1L << 31)) {
+        moao::set_error("serve_submit: invalid system");
+        return MOAO_ERR_ARG;
+    }
+    // R is m x n: the caller states the extent it allocated, and it must be exactly that
+    if (q->R && (q->ldR < r->m || q->R_cols != r->n)) {
+        moao::set_error("serve_submit: R is " + std::to_string(r->m) + " x " +
+                        std::to_string(r->n) + " but the output holds ldR " +
+                        std::to_string(q->ldR) + " x " + std::to_string(q->R_cols) + " columns");
         return MOAO_ERR_ARG;
     }
     r->mode = q->mode;
@@ -273,9 +302,8 @@ int moao_serve_submit(moao_server* s, const moao_serve_request* q, long* ticket)
     r->max_iter = q->max_iter > 0 ? q->max_iter : 5;
     r->R = q->R;
     r->ldR = q->ldR;
-    r->solve_bytes = r->mode == MOAO_FP64
-                         ? moao::al256(moao::recon_invl_doubles(r->n) * 8) + 256
-                         : moao::mixed_workspace_bytes(r->n, r->m, &r->view);
+    r->solve_bytes = r->mode == MOAO_FP64 ? moao::recon_workspace_bytes(r->n)
+                                          : moao::mixed_workspace_bytes(r->n, r->m, &r->view);
     r->footprint = moao::al256(sizeof(double) * r->n * r->n) +
                    moao::al256(sizeof(double) * r->m * r->n) + moao::al256(r->solve_bytes);
     r->res.n = r->n;

@@ -19,9 +19,11 @@ int matcov_tile_launch(double* d_data, int nrows, int ncols, int xoffset, int yo
 size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank);
 
 // Mixed-precision reconstructor (FP32 factorization of C0 = Cmm - lgs_cst U U^T, Woodbury,
-// FP64 refinement to tol).  Cmm is read only; Ctm is overwritten by R.  ws grows as needed.
+// FP64 refinement to tol).  Cmm is read only; Ctm is overwritten by R.  ws: device scratch of
+// at least mixed_workspace_bytes(n, m, lowrank) bytes, owned by the caller.
 int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, double* Ctm, long ldCtm,
                             const moao_tomo_view* lowrank, double tol, int max_iter,
-                            double* resid_out, int* iters_out, DevBuf& ws, hipStream_t s);
+                            double* resid_out, int* iters_out, void* ws, size_t ws_bytes,
+                            hipStream_t s);
 
 }  // namespace moao

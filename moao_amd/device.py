@@ -54,6 +54,24 @@ def synchronize():
     check(lib().moao_hip_synchronize(), "moao_hip_synchronize")
 
 
+class Stream:
+    """A library stream (default priority) for the stream-ordered entries."""
+
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().moao_hip_stream_create(ctypes.byref(p)), "moao_hip_stream_create")
+        self.ptr = p.value
+
+    def synchronize(self):
+        check(lib().moao_hip_stream_synchronize(ctypes.c_void_p(self.ptr)),
+              "moao_hip_stream_synchronize")
+
+    def destroy(self):
+        if self.ptr:
+            lib().moao_hip_stream_destroy(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
 class Profile:
     """Context manager collecting per-kernel-class event timings from the library."""
 

@@ -53,7 +53,8 @@ class ServeRequest(ctypes.Structure):
     """ctypes mirror of moao_serve_request."""
     _fields_ = [("sys_file", ctypes.c_char_p), ("atm_file", ctypes.c_char_p),
                 ("target", ctypes.c_int), ("mode", ctypes.c_int), ("tol", ctypes.c_double),
-                ("max_iter", ctypes.c_int), ("R", _c_double_p), ("ldR", ctypes.c_long)]
+                ("max_iter", ctypes.c_int), ("R", _c_double_p), ("ldR", ctypes.c_long),
+                ("R_cols", ctypes.c_long)]
 
 
 class ServeResult(ctypes.Structure):
@@ -101,6 +102,9 @@ _SIGS = {
     "moao_hip_synchronize": (ctypes.c_int, []),
     "moao_hip_mem_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t),
                                          ctypes.POINTER(ctypes.c_size_t)]),
+    "moao_hip_stream_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "moao_hip_stream_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "moao_hip_stream_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "moao_hip_matcov_tile": (ctypes.c_int, [_c_double_p, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(TomoView), ctypes.c_int]),
@@ -114,6 +118,11 @@ _SIGS = {
     "moao_hip_reconstructor_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_void_p,
                                                   ctypes.c_long, ctypes.c_void_p, ctypes.c_long,
                                                   ctypes.c_long, ctypes.c_void_p]),
+    "moao_hip_reconstructor_async": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
+                                                    ctypes.c_void_p, ctypes.c_long,
+                                                    ctypes.c_void_p, ctypes.c_long,
+                                                    ctypes.c_long, ctypes.c_void_p,
+                                                    ctypes.c_void_p]),
     "moao_hip_reconstructor_mixed_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
                                                         ctypes.c_void_p, ctypes.c_long,
                                                         ctypes.c_void_p, ctypes.c_long,

@@ -30,6 +30,17 @@ def reconstructor_dev(nmeas: int, nmeasts: int, dCmm: int, ldCmm: int, dCtm: int
                  "moao_hip_reconstructor_dev")
 
 
+def reconstructor_async(nmeas: int, nmeasts: int, dCmm: int, ldCmm: int, dCtm: int, ldCtm: int,
+                        ngal: int, d_info: int, stream: int = 0) -> int:
+    """Stream-ordered FP64 solve: returns once queued; the dpotrf status lands in the device int
+    at d_info when the solve runs."""
+    return check(lib().moao_hip_reconstructor_async(nmeas, nmeasts, ctypes.c_void_p(dCmm), ldCmm,
+                                                    ctypes.c_void_p(dCtm), ldCtm, ngal,
+                                                    ctypes.c_void_p(d_info),
+                                                    ctypes.c_void_p(stream)),
+                 "moao_hip_reconstructor_async")
+
+
 def reconstructor_mixed_dev(nmeas: int, nmeasts: int, dCmm: int, ldCmm: int, dCtm: int,
                             ldCtm: int, tomo=None, tol: float = 1e-11, max_iter: int = 5,
                             stream: int = 0, allow_noconv: bool = False):

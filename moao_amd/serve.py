@@ -32,7 +32,8 @@ class Server:
                tol: float = 1e-11, max_iter: int = 5, want_R: bool = True,
                n_m: tuple | None = None) -> int:
         """Queue one refine-step request; returns its ticket.  R (m x n, column-major) is
-        allocated here when want_R; n_m may give its shape, else it is read from the system."""
+        allocated here when want_R; n_m may give its shape, else it is read from the system
+        (the engine refuses an R whose shape is not the system's)."""
         R = None
         if want_R:
             if n_m is None:
@@ -44,7 +45,8 @@ class Server:
         sf, af = sys_file.encode(), atm_file.encode()
         req = ServeRequest(sf, af, target, mode, tol, max_iter,
                            R.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if R is not None
-                           else None, R.shape[0] if R is not None else 0)
+                           else None, R.shape[0] if R is not None else 0,
+                           R.shape[1] if R is not None else 0)
         ticket = ctypes.c_long()
         check(lib().moao_serve_submit(self._h, ctypes.byref(req), ctypes.byref(ticket)),
               "moao_serve_submit")

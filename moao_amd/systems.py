@@ -152,3 +152,28 @@ def dx_matrix(nact: int, nmeasts: int, nmeas: int, seed: int = 7) -> np.ndarray:
     """Command matrix Dx (nact x nmeasts) ~ N(0, 1/nmeas), FP64, column-major."""
     rng = np.random.default_rng(seed)
     return np.asfortranarray(rng.normal(0.0, 1.0 / math.sqrt(nmeas), size=(nact, nmeasts)))
+
+
+def request_stream(outdir: str, nreq: int, seed: int = 2024, n_max: int = 40000):
+    """Systems of a serving stream: each request draws n_wfs ~ U{4..9} measuring sensors (two
+    thirds LGS, the rest NGS, plus the truth sensor) and Nssp ~ U{20..74} on the D = 39 m
+    telescope, redrawn while n = 2 n_wfs Nvalid(Nssp) > n_max, 10 layers with fresh seeded Cn2
+    fractions and r0 ~ U(0.08, 0.2) m.  Returns [(sys_file, atm_file, n, m)], one directory per
+    request under outdir."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    while len(reqs) < nreq:
+        nw = int(rng.integers(4, 10))
+        nssp = int(rng.integers(20, 75))
+        nv = valid_subaps(nssp, 0.28)
+        n = 2 * nw * nv
+        if n > n_max:
+            continue
+        nlgs = max(1, (2 * nw + 2) // 3)
+        spec = SystemSpec(diam=39.0, nssp=nssp, nlgs=nlgs, nngs=nw - nlgs, nlayer=10,
+                          r0=float(rng.uniform(0.08, 0.2)), seed=int(rng.integers(1 << 30)))
+        d = os.path.join(outdir, f"r{len(reqs)}")
+        os.makedirs(d, exist_ok=True)
+        s, atms = make_system(d, spec)
+        reqs.append((s, atms[0], n, 2 * nv))
+    return reqs

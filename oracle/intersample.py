@@ -11,10 +11,16 @@ numpy.fft.rfft2 / irfft2 are FFTW's r2c / c2r transforms (irfft2 * N^2 = FFTW's 
 c2r); _eclat (intersample.cpp:47-70) on an even N is fftshift on both axes.
 N = max(512, 8*2^floor(ln(np)+0.5)) and dactupix = N/(3(np-1)) + 1 (integer division), 175-207.
 
-Pinned by tests/test_oracle_intersample.py: a literal transcription of the _eclat loops equals
-fftshift; the half-spectrum pipeline equals a full complex-FFT evaluation with the Hermitian
-mirror; with Cvv = 0 the PSF peak equals sum(otf_tel) = 1 (diffraction-limited Strehl) and the
-centre of dphi is exp(0) = 1; the Cvv->map scatter equals a direct double loop.
+Pinned by tests/test_oracle_intersample.py against what the mathematics fixes, on inputs that
+are not symmetric under a transpose (so a swapped axis anywhere fails):
+  * a stationary anisotropic Cvv_ab = g(x_a - x_b, y_a - y_b): every map bin equals g at its
+    offset (index orientation, scatter, hit-count division);
+  * white Cvv and a separable Gaussian Cvv with an anisotropic abs2fi: dphi equals the closed
+    form exp(-fact/(2 fact2) (raw(0) - raw)) with raw from direct-sum autocorrelations
+    (fact, fact2, dactupix, centring, abs2fi's [j*N+i] orientation);
+  * PSF pixels by a direct O(N^2) cosine sum with an elliptical OTF, Strehl = sum(otf * dphi);
+  * a literal transcription of the lattice-embedding loop and of the _eclat loops;
+  * the half-spectrum pipeline against full complex FFTs, and Cvv = 0 -> Strehl = sum(otf) = 1.
 """
 from __future__ import annotations
 

@@ -1,0 +1,161 @@
+"""Parity at the full sizes of the benchmark configurations.
+
+  * n = 39,720 (74 x 74 subapertures, 3 LGS + 2 NGS + truth, 10 layers): the FP64 GPU solve and
+    the mixed-precision solve against the oracle's LAPACK path (scipy dpotrf + 2 dtrsm) on the
+    same Cmm/Ctm, and the mixed path's Cee against the FP64 path's Cee;
+  * a serving stream drawn like the request workload (n_wfs ~ U{4..9}, Nssp ~ U{20..74}, n from
+    4.5k to 38.6k, FP64 and mixed requests): every R bit-identical alone and in the mix, every
+    request's backward error ||Ctm - R Cmm||_F / ||Ctm||_F within its bar, and the oracle for
+    the requests with n <= 10k.
+
+Tolerances: FP64 R within 1e-9 of the oracle at n = 39,720 (kappa(Cmm) ~ 1e8 there, so two FP64
+solvers differ at the 1e-10 level; backward error <= 5e-13 is what a stable solver guarantees);
+mixed R and Cee within 1e-6 (the mixed path's specification).
+"""
+import ctypes
+import hashlib
+
+import numpy as np
+import pytest
+
+from moao_amd import systems
+from moao_amd.device import DeviceArray, synchronize
+from moao_amd.lib import MOAO_FP64, MOAO_MIXED_FP32, check, lib
+from moao_amd.matcov import CMM, CTM, CTT, matcov_tile_dev
+from moao_amd.reconstructor import dgemm_dev, reconstructor_dev, reconstructor_mixed_dev
+from moao_amd.serve import Server
+from moao_amd.tomo import Tomo
+from oracle import lapack_path as lp
+from oracle.check import rel_fro
+
+pytestmark = pytest.mark.gpu
+vp = ctypes.c_void_p
+
+
+def _generate(t):
+    n, m = t.nmeas(), t.nmeasts()
+    dC = DeviceArray((n, n))
+    matcov_tile_dev(dC.ptr, n, n, 0, 0, n, t, CMM)
+    t.set_target(0)
+    dB = DeviceArray((m, n))
+    matcov_tile_dev(dB.ptr, m, n, 0, 0, m, t, CTM)
+    synchronize()
+    return dC, dB
+
+
+def _backward_error(dC, dB, R, n, m):
+    """||Ctm - R Cmm||_F / ||Ctm||_F with the product on the device (symmetric Cmm, lower
+    triangle), the norms on the host."""
+    dR = DeviceArray.from_host(R)
+    dE = DeviceArray((m, n))
+    dE.copy_from(dB)
+    dgemm_dev(2, m, n, n, -1.0, dR.ptr, m, dC.ptr, n, 1.0, dE.ptr, m)
+    synchronize()
+    E = dE.to_host()
+    B = dB.to_host()
+    return float(np.linalg.norm(E) / np.linalg.norm(B))
+
+
+def _cee(t, dC, dB, R, n, m, nact=64):
+    """Cee = Ctt - Ctm R^T - R Ctm^T + R Cmm R^T through the Cee/Cvv entry (Cpp, Cpm copies)."""
+    dCpp = DeviceArray((m, m))
+    matcov_tile_dev(dCpp.ptr, m, m, 0, 0, m, t, CTT)
+    dCpm = DeviceArray((m, n))
+    dCpm.copy_from(dB)
+    dR = DeviceArray.from_host(R)
+    Dx = systems.dx_matrix(nact, m, n)
+    dDx = DeviceArray.from_host(Dx)
+    dCee, dCvv, dTmp = DeviceArray((m, m)), DeviceArray((nact, nact)), DeviceArray((nact, m))
+    check(lib().moao_hip_cee_cvv_dev(n, m, nact, vp(dC.ptr), n, vp(dCpp.ptr), m, vp(dCpm.ptr), m,
+                                     vp(dR.ptr), m, vp(dDx.ptr), nact, vp(dCee.ptr), m,
+                                     vp(dCvv.ptr), nact, vp(dTmp.ptr), nact, None),
+          "moao_hip_cee_cvv_dev")
+    synchronize()
+    return dCee.to_host()
+
+
+def test_elt74_fp64_and_mixed_against_oracle(tmp_path):
+    s, atms = systems.make_system(str(tmp_path), "elt74")
+    t = Tomo(s, atms[0])
+    n, m = t.nmeas(), t.nmeasts()
+    print(f"[elt74] n={n} m={m}: matcov on the device", flush=True)
+    dC0, dB0 = _generate(t)
+    C, B = dC0.to_host(), dB0.to_host()
+    print("[elt74] oracle: scipy dpotrf + 2 dtrsm", flush=True)
+    Ro = B.copy(order="F")
+    Co = C.copy(order="F")
+    assert lp.reconstructor(Co, Ro) == 0
+    del Co, C
+    print("[elt74] FP64 solve on the device", flush=True)
+    dC, dB = DeviceArray((n, n)), DeviceArray((m, n))
+    dC.copy_from(dC0)
+    dB.copy_from(dB0)
+    assert reconstructor_dev(n, m, dC.ptr, n, dB.ptr, m, 1) == 0
+    R64 = dB.to_host()
+    dC.free()
+    err64 = rel_fro(R64, Ro)
+    bwd64 = _backward_error(dC0, dB0, R64, n, m)
+    bwdo = _backward_error(dC0, dB0, Ro, n, m)
+    print(f"    FP64: |R - R_lapack|/|R_lapack| = {err64:.2e}, backward {bwd64:.2e} "
+          f"(oracle's own {bwdo:.2e})", flush=True)
+    assert bwd64 <= 5e-13
+    assert err64 <= 1e-9
+    print("[elt74] mixed FP32 solve (refined to 1e-10)", flush=True)
+    dB.copy_from(dB0)
+    rc, res, its = reconstructor_mixed_dev(n, m, dC0.ptr, n, dB.ptr, m, t, 1e-10, 6)
+    Rmx = dB.to_host()
+    errmx = rel_fro(Rmx, Ro)
+    print(f"    mixed: err={errmx:.2e} vs oracle, {rel_fro(Rmx, R64):.2e} vs FP64 GPU, "
+          f"resid={res:.2e}, steps={its}", flush=True)
+    assert rc == 0 and errmx <= 1e-6
+    cee64 = _cee(t, dC0, dB0, R64, n, m)
+    ceemx = _cee(t, dC0, dB0, Rmx, n, m)
+    ecee = rel_fro(ceemx, cee64)
+    print(f"    Cee(mixed) vs Cee(FP64): {ecee:.2e}", flush=True)
+    assert ecee <= 1e-6
+
+
+def test_serving_stream_of_uneven_requests(tmp_path):
+    reqs = systems.request_stream(str(tmp_path), 8, seed=5)
+    modes = [MOAO_FP64 if k % 3 != 2 else MOAO_MIXED_FP32 for k in range(len(reqs))]
+    print("[stream] (n, m, mode):", [(n, m, md) for (_, _, n, m), md in zip(reqs, modes)],
+          flush=True)
+
+    def digest(a):
+        return hashlib.sha256(np.ascontiguousarray(a).view(np.uint8)).hexdigest()
+
+    keep = {k for k, (_, _, n, _) in enumerate(reqs) if n <= 10000}
+    mix_hash, mix_R, mix_res = {}, {}, {}
+    with Server(slots=4) as srv:
+        tickets = [srv.submit(s, a, 0, md, tol=1e-11, max_iter=8, n_m=(n, m))
+                   for (s, a, n, m), md in zip(reqs, modes)]
+        for k, tk in enumerate(tickets):
+            r = srv.wait(tk)
+            mix_hash[k] = digest(r["R"])
+            mix_res[k] = r
+            if r["status"] == 0:
+                mix_R[k] = r["R"]
+            print(f"    mix request {k}: status={r['status']} latency="
+                  f"{r['latency_s'] * 1e3:.0f} ms slot={r['slot']}", flush=True)
+        st = srv.stats()
+    assert st["peak_running"] >= 2
+    with Server(slots=1) as srv:
+        for k, ((s, a, n, m), md) in enumerate(zip(reqs, modes)):
+            r = srv.wait(srv.submit(s, a, 0, md, tol=1e-11, max_iter=8, n_m=(n, m)))
+            assert digest(r["R"]) == mix_hash[k], f"request {k} differs in the mix"
+    for k, ((s, a, n, m), md) in enumerate(zip(reqs, modes)):
+        t = Tomo(s, a)
+        dC, dB = _generate(t)
+        bwd = _backward_error(dC, dB, mix_R[k], n, m)
+        line = f"    request {k} n={n} m={m} mode={md}: backward {bwd:.2e}"
+        assert bwd <= (5e-13 if md == MOAO_FP64 else 2e-11)
+        if k in keep:
+            Ro = dB.to_host()
+            assert lp.reconstructor(dC.to_host(), Ro) == 0
+            err = rel_fro(mix_R[k], Ro)
+            line += f", vs oracle {err:.2e}"
+            assert err <= (1e-10 if md == MOAO_FP64 else 1e-6)
+        print(line, flush=True)
+        dC.free()
+        dB.free()
+        del mix_R[k]

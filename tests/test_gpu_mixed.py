@@ -50,12 +50,45 @@ def test_mixed_small_workload(tmp_path):
     assert res <= 1e-11 and err <= 1e-6
 
 
-def test_mixed_elt_workload(tmp_path):
-    """n = 18,688 (the 40x40, 6 LGS + 2 NGS system): the FP32 factorization and solves run on
-    the 128-row FP32 kernels and every FP64 residual on the symmetric direct-to-LDS kernel."""
+@pytest.fixture(scope="module")
+def elt_system(tmp_path_factory):
+    """n = 18,688 (the 40x40, 6 LGS + 2 NGS, 10-layer system): Cmm, Ctm from the library's
+    matcov and the oracle's R on them."""
+    d = tmp_path_factory.mktemp("elt")
+    s, atms = systems.make_system(str(d), "elt")
+    t = Tomo(s, atms[0])
+    C = gm.cmm(t)
+    t.set_target(0)
+    B = gm.ctm(t)
+    Ro = B.copy(order="F")
+    assert lp.reconstructor(C.copy(order="F"), Ro) == 0
+    return t, C, B, Ro
+
+
+def test_fp64_elt_workload(elt_system):
+    """The plain FP64 GPU solve of the n = 18,688 system against the oracle."""
+    from moao_amd.reconstructor import reconstructor
+    print("[fp64] seeded elt workload (n=18688)", flush=True)
+    t, C, B, Ro = elt_system
+    n, m = C.shape[0], B.shape[0]
+    Cg, Bg = C.copy(order="F"), B.copy(order="F")
+    assert reconstructor(n, m, Cg, n, Bg, m, 1) == 0
+    err = rel_fro(Bg, Ro)
+    print(f"    err={err:.2e}", flush=True)
+    assert err <= 1e-10
+
+
+def test_mixed_elt_workload(elt_system):
+    """n = 18,688: the FP32 factorization and solves run on the 128-row FP32 kernels and every
+    FP64 residual on the symmetric direct-to-LDS kernel."""
     print("[mixed] seeded elt workload (n=18688)", flush=True)
-    s, atms = systems.make_system(str(tmp_path), "elt")
-    err, res, its = _run(Tomo(s, atms[0]))
+    t, C, B, Ro = elt_system
+    n, m = C.shape[0], B.shape[0]
+    dC, dB = DeviceArray.from_host(C), DeviceArray.from_host(B)
+    rc, res, its = reconstructor_mixed_dev(n, m, dC.ptr, n, dB.ptr, m, t, 1e-11, 6)
+    R = dB.to_host()
+    assert np.array_equal(dC.to_host(), C)
+    err = rel_fro(R, Ro)
     print(f"    err={err:.2e} resid={res:.2e} steps={its}", flush=True)
     assert res <= 1e-11 and err <= 1e-6
 

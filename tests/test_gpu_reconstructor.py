@@ -100,23 +100,99 @@ def test_large_sizes(n, m):
 
 
 def test_leading_dims_and_targets():
-    """ldCmm > n, ldCtm > m and two targets stored n*m apart (reconstructor.cpp:43)."""
-    print("[recon] leading dims + 2 targets", flush=True)
+    """ldCmm > n on a buffer of exactly LAPACK's minimum extent ((n-1)*ldCmm + n doubles), and
+    two targets stored n*m apart (reconstructor.cpp:43)."""
+    print("[recon] ldCmm > n (minimum buffer) + 2 targets", flush=True)
     rng = np.random.default_rng(5)
-    n, m, ld_a, ld_b = 300, 40, 310, 40
+    n, m, ld_a = 300, 40, 310
     G = rng.standard_normal((n, n))
     S = G @ G.T / n + np.eye(n)
-    Cbuf = np.zeros((ld_a, n), order="F")
-    Cbuf[:n] = np.tril(S)
+    Cbuf = np.zeros((n - 1) * ld_a + n)
+    for c in range(n):
+        Cbuf[c * ld_a + c:c * ld_a + n] = S[c:, c]
+    Cbuf[[c * ld_a + r for c in range(n - 1) for r in range(n, ld_a)]] = 7.0  # row padding
     B0, B1 = rng.standard_normal((m, n)), rng.standard_normal((m, n))
     Bbuf = np.concatenate([np.asfortranarray(B0).reshape(-1, order="F"),
                            np.asfortranarray(B1).reshape(-1, order="F")])
-    info = reconstructor(n, m, Cbuf, ld_a, Bbuf, ld_b, 2)
+    info = reconstructor(n, m, Cbuf, ld_a, Bbuf, m, 2)
     assert info == 0
     for g, Bg in enumerate((B0, B1)):
         R = Bbuf[g * n * m:(g + 1) * n * m].reshape((m, n), order="F")
         assert np.linalg.norm(Bg - R @ S) / np.linalg.norm(Bg) <= 5e-13
-    assert np.all(Cbuf[n:] == 0.0)
+    # padding rows between the columns come back unchanged
+    assert np.all(Cbuf[[c * ld_a + r for c in range(n - 1) for r in range(n, ld_a)]] == 7.0)
+
+
+def test_ldctm_greater_than_m():
+    """One target with ldCtm > m on a buffer of exactly (n-1)*ldCtm + m doubles."""
+    print("[recon] ldCtm > m (minimum buffer)", flush=True)
+    rng = np.random.default_rng(6)
+    n, m, ld_b = 260, 30, 37
+    G = rng.standard_normal((n, n))
+    S = G @ G.T / n + np.eye(n)
+    C = np.asfortranarray(np.tril(S))
+    B = rng.standard_normal((m, n))
+    Bbuf = np.full((n - 1) * ld_b + m, 5.0)
+    for c in range(n):
+        Bbuf[c * ld_b:c * ld_b + m] = B[:, c]
+    assert reconstructor(n, m, C, n, Bbuf, ld_b, 1) == 0
+    R = np.stack([Bbuf[c * ld_b:c * ld_b + m] for c in range(n)], axis=1)
+    assert np.linalg.norm(B - R @ S) / np.linalg.norm(B) <= 5e-13
+    assert np.all(Bbuf[[c * ld_b + r for c in range(n - 1) for r in range(m, ld_b)]] == 5.0)
+
+
+def test_overlapping_targets_are_rejected():
+    """With ngal > 1 the blocks start g*n*m apart; ldCtm > m would make them overlap."""
+    from moao_amd.lib import MOAO_ERR_ARG, MoaoError
+    n, m = 64, 8
+    C = np.asfortranarray(np.eye(n))
+    B = np.zeros(2 * n * 10)
+    with pytest.raises(MoaoError) as e:
+        reconstructor(n, m, C, n, B, 10, 2)
+    assert e.value.code == MOAO_ERR_ARG
+
+
+def _partial_factor_ok(Cg, S, ncols):
+    """The first ncols columns of the GPU factor equal the Cholesky factor's columns, computed
+    from the leading ncols x ncols block (numpy) and the rows below by a triangular solve."""
+    L11 = np.linalg.cholesky(S[:ncols, :ncols])
+    L21 = np.linalg.solve(L11, S[ncols:, :ncols].T).T
+    ref = np.vstack([L11, L21])
+    got = np.tril(Cg[:, :ncols])
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-12
+
+
+def test_two_streams_overlap():
+    """Two FP64 solves queued on two streams with the stream-ordered entry (their kernels may
+    overlap on the device): each R is bit-identical to the same solve run alone."""
+    from moao_amd.device import DeviceArray, Stream, synchronize
+    from moao_amd.reconstructor import reconstructor_async, reconstructor_dev
+    print("[recon] two streams", flush=True)
+    rng = np.random.default_rng(21)
+    probs = []
+    for n, m in [(3000, 400), (2200, 700)]:
+        G = rng.standard_normal((n, n))
+        C = np.asfortranarray(np.tril(G @ G.T / n + np.eye(n)))
+        B = np.asfortranarray(rng.standard_normal((m, n)))
+        probs.append((n, m, C, B))
+    solo = []
+    for n, m, C, B in probs:
+        dC, dB = DeviceArray.from_host(C), DeviceArray.from_host(B)
+        assert reconstructor_dev(n, m, dC.ptr, n, dB.ptr, m, 1) == 0
+        solo.append(dB.to_host())
+    s1, s2 = Stream(), Stream()
+    dev = [(DeviceArray.from_host(C), DeviceArray.from_host(B)) for _, _, C, B in probs]
+    info = DeviceArray((2,), np.int32)
+    synchronize()
+    for k, ((n, m, _, _), (dC, dB), st) in enumerate(zip(probs, dev, (s1, s2))):
+        reconstructor_async(n, m, dC.ptr, n, dB.ptr, m, 1, info.ptr + 4 * k, st.ptr)
+    s1.synchronize()
+    s2.synchronize()
+    assert list(info.to_host()) == [0, 0]
+    for (dC, dB), R in zip(dev, solo):
+        assert np.array_equal(dB.to_host(), R)
+    s1.destroy()
+    s2.destroy()
 
 
 def test_not_positive_definite_reports_info():
@@ -130,9 +206,14 @@ def test_not_positive_definite_reports_info():
     Co = C.copy(order="F")
     info_ref = lp.reconstructor(Co, B.copy(order="F"))
     Bg = B.copy(order="F")
-    info = reconstructor(n, m, C.copy(order="F"), n, Bg, m, 1)
+    Cg = C.copy(order="F")
+    info = reconstructor(n, m, Cg, n, Bg, m, 1)
     assert info == info_ref == 301
     assert np.array_equal(Bg, B)  # R not computed: Ctm untouched on the host entry
+    # column 300 is local column 44 of the leaf [256, 384): the leaves before it and that
+    # leaf's 16-column blocks before column 288 hold the factor
+    S = np.tril(C) + np.tril(C, -1).T
+    _partial_factor_ok(Cg, S, 288)
 
 
 def test_nan_pivot_reports_info():
@@ -149,5 +230,9 @@ def test_nan_pivot_reports_info():
     C[200, 150] = np.nan
     B = np.asfortranarray(rng.standard_normal((m, n)))
     Bg = B.copy(order="F")
-    assert reconstructor(n, m, C.copy(order="F"), n, Bg, m, 1) == 201
+    Cg = C.copy(order="F")
+    assert reconstructor(n, m, Cg, n, Bg, m, 1) == 201
     assert np.array_equal(Bg, B)
+    # leaf [128, 256): its blocks before column 192 (the one holding column 200) are stored
+    S = np.tril(C) + np.tril(C, -1).T
+    _partial_factor_ok(Cg, S, 150)

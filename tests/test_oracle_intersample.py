@@ -83,3 +83,198 @@ def test_half_spectrum_equals_full_complex_fft():
     ref = np.fft.fftshift(np.fft.ifft2(G).real * N * N)
     assert np.max(np.abs(psf - ref)) <= 1e-12 * np.max(np.abs(ref))
     assert 0.0 < psf[N // 2, N // 2] < 1.0
+
+
+# ---------------------------------------------------------------------------------------------
+# Pins from what the mathematics fixes.  Every input below is anisotropic (not symmetric under a
+# transpose), so an oracle that swapped rows and columns anywhere -- the Cvv-to-map index, the
+# lattice embedding, abs2fi's [j*N + i] indexing, the OTF's -- fails them.
+#
+# Conventions read from the reference's text (common/tomo_struct/intersample.cpp):
+#   Map[idx[k]-1] += Cvv[k], /= hit count                                        83-95
+#   map[j + i*N] = Map[cpt], i (slow) and j (fast) on the dactupix lattice        98-111
+#   tmp[i*(N/2+1)+j] *= abs2fi[j*N+i] after the r2c of map (i: first dimension)   115-121
+#   fact = 2 (2*3.14159/lambda)^2, fact2 = N^2 dactupix^2, lambda = 1.65e-6       131-135
+#   tmp[i*(N/2+1)+j] = otf_tel[j*N+i] * dphi[j*N+i]; c2r; _eclat                 149-167
+# and idx from the reference's generator (cython/pipeline/generate_file.py:57-76):
+#   Cvv[a + b*nact] -> bin (dx - 1) + (2Np-1)(dy - 1), dx = x_a - x_b + Np, dy = y_a - y_b + Np,
+#   x = column and y = row of the actuator in the Ndiam x Ndiam grid.
+
+def _grid_xy(dm):
+    return dm.csI % dm.ndiam, dm.csI // dm.ndiam
+
+
+def _circ_autocorr(f):
+    """A(s) = sum_t f[t] f[(t + s) mod N], by direct sums (no FFT)."""
+    N = f.size
+    return np.array([np.dot(f, np.roll(f, -s)) for s in range(N)])
+
+
+def _gauss1d(N, width_pix):
+    t = np.arange(N) - N // 2
+    return np.exp(-t * t / (2.0 * width_pix * width_pix))
+
+
+def _abs2fi_separable(N, wr, wc):
+    """abs2fi in the reference's flat layout for a separable influence function: frequency
+    (row i, column j) of the map's transform gets Pr[i] Pc[j] (abs2fi[j*N + i])."""
+    Pr = np.abs(np.fft.fft(_gauss1d(N, wr))) ** 2
+    Pc = np.abs(np.fft.fft(_gauss1d(N, wc))) ** 2
+    H = np.outer(Pr, Pc)                     # H[i][j]
+    return H.T.flatten(), _gauss1d(N, wr), _gauss1d(N, wc)
+
+
+def _elliptic_otf(N, ax, ay):
+    """A centrosymmetric, transpose-asymmetric OTF (autocorrelation of an elliptical pupil),
+    normalised to sum 1, in the reference's flat layout (the array itself, flattened)."""
+    c = (np.arange(N) - N // 2) / (N / 6.0)
+    Y, X = np.meshgrid(c, c, indexing="ij")
+    pup = ((X / ax) ** 2 + (Y / ay) ** 2 <= 1.0).astype(float)
+    ac = np.fft.ifft2(np.abs(np.fft.fft2(pup)) ** 2).real
+    return (ac / ac.sum()).flatten()
+
+
+def test_stationary_anisotropic_cvv_fills_the_map_with_g():
+    """Cvv_ab = g(x_a - x_b, y_a - y_b) with an anisotropic, odd-in-x g: every bin the index
+    hits holds exactly g at that offset (all its entries are equal), bins never hit stay 0.
+    This pins the index (dx fast, dy slow), the scatter and the hit-count division."""
+    dm = pi.make_dm(10, 0.2, 0.28)
+    np_ = dm.ndiam
+    ncov = 2 * np_ - 1
+    x, y = _grid_xy(dm)
+
+    def g(dx, dy):
+        return np.exp(-dx * dx / 18.0 - dy * dy / 2.0) + 0.05 * dx + 0.2
+
+    C = g(x[:, None] - x[None, :], y[:, None] - y[None, :])       # C[a][b]
+    idx = pi.make_idx(dm)
+    Map = oi.cvv_map(C.flatten(order="F"), idx, np_)              # Cvv flat a + b*nact
+    hit = np.bincount(idx - 1, minlength=ncov * ncov) > 0
+    dxs, dys = np.meshgrid(np.arange(ncov) - (np_ - 1), np.arange(ncov) - (np_ - 1))
+    expect = g(dxs, dys).flatten()                                # bin dx + ncov*dy
+    assert np.allclose(Map[hit], expect[hit], rtol=1e-14, atol=0)
+    assert np.all(Map[~hit] == 0.0)
+    M2 = Map.reshape(ncov, ncov)
+    assert not np.allclose(M2, M2.T)                              # orientation is visible
+
+
+def test_embedding_is_the_reference_lattice():
+    """A literal transcription of the embedding loop (intersample.cpp:98-111) equals the
+    oracle's, and the zero offset lands on pixel (N/2, N/2) (ncmap - 1), dx = +1 one lattice
+    step along the fast (column) index."""
+    dm = pi.make_dm(10, 0.2, 0.28)
+    np_ = dm.ndiam
+    ncov = 2 * np_ - 1
+    N = 512
+    dp = int(oi.dactupix(N, np_))
+    Map = np.arange(1.0, ncov * ncov + 1.0)
+    flat = np.zeros(N * N)
+    ncmap, nelem = N // 2 + 1, (ncov - 1) // 2
+    cpt = 0
+    for i in range(ncmap - nelem * dp - 1, ncmap + nelem * dp, dp):
+        for j in range(ncmap - nelem * dp - 1, ncmap + nelem * dp, dp):
+            flat[j + i * N] = Map[cpt]
+            cpt += 1
+    m = oi.embed(Map, N, np_)
+    assert np.array_equal(m.flatten(), flat)
+    centre = nelem * ncov + nelem
+    assert m[N // 2, N // 2] == Map[centre]
+    assert m[N // 2, N // 2 + dp] == Map[centre + 1]        # dx = +1
+    assert m[N // 2 + dp, N // 2] == Map[centre + ncov]     # dy = +1
+
+
+def _dphi_closed_form(N, dp, M, pos, Ar, Ac):
+    """exp(-D/2) before _eclat for the covariance map M (rows = dy) on lattice positions pos:
+    raw[i][j] = N^2 sum_{a,b} M[a][b] Ar(i - pos_a) Ac(j - pos_b)  (direct sums, circular)."""
+    Ra = Ar[(np.arange(N)[:, None] - pos[None, :]) % N]     # Ra[i][a]
+    Cb = Ac[(np.arange(N)[:, None] - pos[None, :]) % N]     # Cb[j][b]
+    raw = N * N * (Ra @ M @ Cb.T)
+    fact = 2 * (2 * 3.14159 / 1.65e-6) ** 2
+    fact2 = N * N * dp * dp
+    return np.exp(-0.5 * (raw[N // 2, N // 2] / fact2 * fact - raw / fact2 * fact))
+
+
+def test_white_cvv_closed_form():
+    """Cvv = s2 I puts s2 on the centre bin only, so the filtered map is s2 times the
+    autocorrelation of the influence function centred at (N/2, N/2):
+        raw(i, j) = N^2 s2 Ar(i - N/2) Ac(j - N/2)       (FFTW's unnormalised c2r)
+    with Ar / Ac the direct-sum autocorrelations of the row / column factors of abs2fi, and
+    dphi = _eclat(exp(-0.5 fact/fact2 (raw(N/2, N/2) - raw))).  The row and column widths
+    differ, which fixes abs2fi's [j*N + i] orientation; the amplitude fixes fact, fact2 and
+    dactupix."""
+    dm = pi.make_dm(10, 0.2, 0.28)
+    np_ = dm.ndiam
+    N = 512
+    dp = oi.dactupix(N, np_)
+    a2f, fr, fc = _abs2fi_separable(N, 6.0, 2.5)
+    s2 = 3e-14
+    cvv = (s2 * np.eye(dm.nact)).flatten(order="F")
+    otf = _elliptic_otf(N, 1.0, 0.6)
+    psf, dphi = oi.process(cvv, pi.make_idx(dm), a2f, otf, N, np_, return_dphi=True)
+    ncov = 2 * np_ - 1
+    M = np.zeros((ncov, ncov))
+    M[np_ - 1, np_ - 1] = s2
+    pos = N // 2 + (np.arange(ncov) - (np_ - 1)) * int(dp)
+    E = _dphi_closed_form(N, dp, M, pos, _circ_autocorr(fr), _circ_autocorr(fc))
+    assert np.allclose(dphi, np.fft.fftshift(E), rtol=1e-10, atol=1e-13)
+    assert 0.05 < E.min() < 0.95                       # the phase term is far from trivial
+    assert not np.allclose(E, E.T, rtol=1e-3)
+
+
+def test_separable_stationary_cvv_closed_form():
+    """Cvv_ab = gy(y_a - y_b) gx(x_a - x_b) (separable Gaussian covariance, positive definite)
+    fills the map with gy(dy) gx(dx) on the bins the index hits (rows = dy), so
+        raw(i, j) = N^2 sum_{a,b} M[a][b] Ar(i - pos_a) Ac(j - pos_b)
+    by direct sums.  gx, gy and the abs2fi factors are all anisotropic: the map's row axis must
+    meet abs2fi's row factor."""
+    dm = pi.make_dm(10, 0.2, 0.28)
+    np_ = dm.ndiam
+    ncov = 2 * np_ - 1
+    N = 512
+    dp = int(oi.dactupix(N, np_))
+    a2f, fr, fc = _abs2fi_separable(N, 5.0, 2.0)
+    x, y = _grid_xy(dm)
+    sx, sy, amp = 3.0, 1.2, 2e-14
+    C = amp * np.exp(-(x[:, None] - x[None, :]) ** 2 / (2 * sx * sx)
+                     - (y[:, None] - y[None, :]) ** 2 / (2 * sy * sy))
+    idx = pi.make_idx(dm)
+    otf = _elliptic_otf(N, 0.7, 1.0)
+    psf, dphi = oi.process(C.flatten(order="F"), idx, a2f, otf, N, np_, return_dphi=True)
+    # the map is the separable g on every bin the index hits, 0 on the rest (the corners of
+    # the annular DM's offset range)
+    hit = (np.bincount(idx - 1, minlength=ncov * ncov) > 0).reshape(ncov, ncov)
+    offs = np.arange(ncov) - (np_ - 1)
+    M = amp * np.outer(np.exp(-offs ** 2 / (2 * sy * sy)), np.exp(-offs ** 2 / (2 * sx * sx)))
+    M[~hit] = 0.0
+    pos = N // 2 + offs * dp
+    E = _dphi_closed_form(N, dp, M, pos, _circ_autocorr(fr), _circ_autocorr(fc))
+    assert np.allclose(dphi, np.fft.fftshift(E), rtol=1e-10, atol=1e-13)
+    assert not np.allclose(E, E.T, rtol=1e-3)
+
+
+def test_psf_pixels_by_direct_dft():
+    """PSF = _eclat(c2r(G)) with G[i][j] = otf_tel[j*N+i] dphi[j*N+i] (intersample.cpp:149-167):
+    G is real and centrosymmetric, so the c2r is the plain cosine sum; a few pixels by direct
+    O(N^2) sums, and the centre (Strehl) = sum(otf * dphi)."""
+    dm = pi.make_dm(10, 0.2, 0.28)
+    np_ = dm.ndiam
+    N = 256
+    a2f, _, _ = _abs2fi_separable(N, 3.0, 1.5)
+    rng = np.random.default_rng(4)
+    Gm = rng.standard_normal((dm.nact, dm.nact))
+    cvv = (Gm @ Gm.T / dm.nact * 2e-14).flatten(order="F")
+    otf = _elliptic_otf(N, 1.0, 0.55)
+    psf, dphi = oi.process(cvv, pi.make_idx(dm), a2f, otf, N, np_, return_dphi=True)
+    dflat = dphi.flatten()
+    G = (otf * dflat).reshape(N, N).T          # G[i][j] = otf[j*N+i] dphi[j*N+i]
+    assert np.allclose(G, G[(-np.arange(N)) % N][:, (-np.arange(N)) % N], rtol=1e-9, atol=1e-16)
+    i = np.arange(N)[:, None]
+    j = np.arange(N)[None, :]
+    for p, q in [(N // 2, N // 2), (N // 2 + 3, N // 2), (N // 2, N // 2 + 3), (10, 200),
+                 (N // 2 - 7, N // 2 + 1)]:
+        pp, qq = (p + N // 2) % N, (q + N // 2) % N   # _eclat
+        ref = np.sum(G * np.cos(2 * np.pi * (i * pp + j * qq) / N))
+        assert psf[p, q] == pytest.approx(ref, rel=1e-9, abs=1e-12 * abs(psf).max())
+    assert psf[N // 2, N // 2] == pytest.approx(np.sum(otf * dflat), rel=1e-12)
+    # the elliptical OTF makes the PSF itself transpose-asymmetric
+    assert abs(psf[N // 2 + 3, N // 2] - psf[N // 2, N // 2 + 3]) > 1e-3 * psf[N // 2, N // 2]
