@@ -139,6 +139,17 @@ int moao_hip_reconstructor_dev(long nmeas, long nmeasts, double* dCmm, long ldCm
 int moao_hip_reconstructor_async(long nmeas, long nmeasts, double* dCmm, long ldCmm,
                                  double* dCtm, long ldCtm, long ngal, int* d_info, void* stream);
 
+/* The same solve on FP32 storage: MOAO_CHAMELEON::reconstructor<float>, the precision the
+ * reference's Chameleon driver runs in production (chameleon/moao_chameleon.cpp:41,
+ * chameleon/compute/reconstructor.cpp:31-65: spotrf + 2 strsm).  FP32 MFMA updates, the
+ * diagonal 128 x 128 blocks factored in FP64 and rounded to FP32.  Status as the FP64 entry
+ * (> 0: the FP32 factorization met a non-positive pivot, which FP32 does on ill-conditioned
+ * Cmm, as spotrf would). */
+int moao_hip_sreconstructor(long nmeas, long nmeasts, float* Cmm, long ldCmm, float* Ctm,
+                            long ldCtm, long ngal);
+int moao_hip_sreconstructor_dev(long nmeas, long nmeasts, float* dCmm, long ldCmm, float* dCtm,
+                                long ldCtm, long ngal, void* stream);
+
 /* Mixed precision (no reference counterpart; same result contract as the FP64 entry to a
  * stated tolerance).  Cmm is read only (lower triangle) and preserved; the Ctm block (one
  * target) is overwritten by R.  If `lowrank` describes the system Cmm was generated from, the
@@ -184,6 +195,48 @@ int moao_hip_intersample_process(moao_isample* s, const double* Cvv, double* psf
 int moao_hip_intersample_process_dev(moao_isample* s, const double* dCvv, int batch,
                                      double* dPsf, void* stream);
 void moao_hip_intersample_destroy(moao_isample* s);
+
+/* The reference's own intersample boundary, same names, arguments and results
+ * (common/tomo_struct/intersample.hpp:22-63, intersample.cpp:175-333):
+ *   prepare  N = max(512, 8*2^(long)(ln(np)+0.5)), dactupix = N/(3(np-1))+1 (long division),
+ *            lambda = 1.65e-6, file names path/idx.fits, path/abs2fi.fits, path/otftel.fits;
+ *            Dtel is ignored as in the reference;
+ *   init     reads the three FITS files (first axis checked against nidx / N, as the
+ *            reference does) and sets up the device stage;
+ *   process  the PSF of Cvv (nidx values, double or float) is left in dphi (N*N doubles);
+ *   free     releases the arrays and the device stage.
+ * The struct keeps the reference's public fields; FFTW's plans and scratch are replaced by the
+ * device stage `dev`.  precision (set by prepare to 1 = FP64 transforms, the reference's
+ * arithmetic) may be set to 0 before init for the FP32-transform product path.
+ * Returns MOAO_* status codes (the C++ layer in moao_hip.hpp maps them to the reference's
+ * 1 = success / 0 = failure). */
+typedef struct moao_isample_struct {
+    long N;            /* size of the FFT support */
+    long np;           /* actuators across the diameter (Nssp of the truth sensor + 1) */
+    long nidx;         /* entries of the index map (nact * nact) */
+    double lambda;     /* wavelength */
+    double dactupix;   /* inter-actuator distance in pixels */
+    long* idx;         /* [nidx] 1-based map indices (read by init) */
+    double* abs2fi;    /* [N*N] */
+    double* otf_tel;   /* [N*N] */
+    double* dphi;      /* [N*N] the PSF after process */
+    char filename_idx[256], filename_abs2fi[256], filename_otftel[256];
+    int precision;     /* 1 = FP64 transforms (default), 0 = FP32 transforms */
+    moao_isample* dev; /* device stage (init) */
+} moao_isample_struct;
+void moao_hip_intersample_prepare(moao_isample_struct* s, long nidx, long np, float Dtel,
+                                  const char* path);
+int moao_hip_intersample_init(moao_isample_struct* s);
+int moao_hip_intersample_process_d(moao_isample_struct* s, const double* Cvv);
+int moao_hip_intersample_process_f(moao_isample_struct* s, const float* Cvv);
+void moao_hip_intersample_free(moao_isample_struct* s);
+
+/* ---------------------------------------------------------------- FITS images -------------
+ * The reference's FITS layout (common/io/fits.cpp:66-148): a primary image, NAXIS1 the fast
+ * axis, memory order on disk.  read: any of BITPIX -64/-32/8/16/32/64 as doubles; out == NULL
+ * queries the axes only (cap = capacity of out, in values).  write: doubles (writeFits). */
+int moao_fits_read(const char* path, long* nx, long* ny, double* out, long cap);
+int moao_fits_write(const char* path, long nx, long ny, const double* data);
 
 /* ---------------------------------------------------------------- GEMM utility ------------
  * C = alpha*A*op(B) + beta*C on the MFMA engine the stages use (column-major);

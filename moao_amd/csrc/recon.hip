@@ -290,6 +290,27 @@ int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm
     return MOAO_OK;
 }
 
+int reconstructor_f32_dev(long n, long m, float* dCmm, long ldCmm, float* dCtm, long ldCtm,
+                          long ngal, float* dinvL, int* dinfo, hipStream_t stream) {
+    if (n < 1 || m < 0 || ngal < 1 || ldCmm < n || (m > 0 && ldCtm < m) || !dCmm || !dinvL ||
+        !dinfo || (m > 0 && !dCtm) || n >= (1L << 31) || m >= (1L << 31)) {
+        set_error("reconstructor (FP32): invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    if (ngal > 1 && m > 0 && ldCtm != m) {
+        set_error("reconstructor (FP32): several targets need ldCtm == nmeasts");
+        return MOAO_ERR_ARG;
+    }
+    std::vector<RowGroup<float>> groups;
+    if (m > 0)
+        for (long g = 0; g < ngal; ++g) groups.push_back({dCtm + g * n * m, ldCtm, (int)m});
+    if (int rc = potrf_tall<float>(n, dCmm, ldCmm, groups, dinvL, dinfo, stream)) return rc;
+    if (m > 0)
+        if (int rc = trsm_right_lower_notrans<float>(n, dCmm, ldCmm, dinvL, groups, stream, dinfo))
+            return rc;
+    return MOAO_OK;
+}
+
 size_t recon_workspace_bytes(long n) {
     return ((recon_invl_doubles(n) * sizeof(double) + 255) & ~size_t(255)) + 256;
 }
@@ -392,6 +413,84 @@ int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, do
         MOAO_HIP_TRY(hipMemcpyAsync(Ctm, dB, b_elems * sizeof(double), hipMemcpyDeviceToHost, s));
     MOAO_HIP_TRY(hipStreamSynchronize(s));
     return hinfo;
+}
+
+// FP32 storage: the Chameleon production precision (chameleon/moao_chameleon.cpp:41 runs
+// MOAO_CHAMELEON::reconstructor<float>: spotrf + 2 strsm on the same maths).  Updates on FP32
+// MFMA, leaves factored in FP64 and rounded to FP32.
+int moao_hip_sreconstructor_dev(long nmeas, long nmeasts, float* dCmm, long ldCmm, float* dCtm,
+                                long ldCtm, long ngal, void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (nmeas < 1) {
+        moao::set_error("sreconstructor: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    hipStream_t s = moao::pick_stream(stream);
+    char* ws = nullptr;
+    const size_t bytes = moao::recon_workspace_bytes(nmeas);  // doubles' room holds the floats
+    if (hipMallocAsync(reinterpret_cast<void**>(&ws), bytes, s) != hipSuccess) {
+        (void)hipGetLastError();
+        moao::set_error("sreconstructor: workspace allocation failed");
+        return MOAO_ERR_NOMEM;
+    }
+    int* info = reinterpret_cast<int*>(ws + bytes - 256);
+    int rc = moao::reconstructor_f32_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, ngal,
+                                         reinterpret_cast<float*>(ws), info, s);
+    int hinfo = 0;
+    if (rc == MOAO_OK &&
+        hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) {
+        moao::set_error("sreconstructor: status copy failed");
+        rc = MOAO_ERR_HIP;
+    }
+    (void)hipFreeAsync(ws, s);
+    MOAO_HIP_TRY(hipStreamSynchronize(s));
+    return rc != MOAO_OK ? rc : hinfo;
+}
+
+int moao_hip_sreconstructor(long nmeas, long nmeasts, float* Cmm, long ldCmm, float* Ctm,
+                            long ldCtm, long ngal) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (!Cmm || (nmeasts > 0 && !Ctm) || nmeas < 1 || nmeasts < 0 || ngal < 1 ||
+        ldCmm < nmeas || (nmeasts > 0 && ldCtm < nmeasts)) {
+        moao::set_error("sreconstructor: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    hipStream_t s = moao::ctx().stream;
+    const size_t a_bytes = ((size_t)(nmeas - 1) * ldCmm + nmeas) * sizeof(float);
+    const size_t b_elems = nmeasts > 0 ? (size_t)(ngal - 1) * nmeas * nmeasts +
+                                             (size_t)(nmeas - 1) * ldCtm + nmeasts
+                                       : 0;
+    char* d = nullptr;
+    const size_t a_al = (a_bytes + 255) & ~size_t(255);
+    if (hipMallocAsync(reinterpret_cast<void**>(&d), a_al + b_elems * sizeof(float) + 256, s) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        moao::set_error("sreconstructor: staging allocation failed");
+        return MOAO_ERR_NOMEM;
+    }
+    float* dA = reinterpret_cast<float*>(d);
+    float* dB = reinterpret_cast<float*>(d + a_al);
+    int rc = MOAO_OK;
+    if (hipMemcpyAsync(dA, Cmm, a_bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        (b_elems && hipMemcpyAsync(dB, Ctm, b_elems * sizeof(float), hipMemcpyHostToDevice, s) !=
+                        hipSuccess)) {
+        moao::set_error("sreconstructor: upload failed");
+        rc = MOAO_ERR_HIP;
+    }
+    if (rc == MOAO_OK) rc = moao_hip_sreconstructor_dev(nmeas, nmeasts, dA, ldCmm, dB, ldCtm, ngal, s);
+    if (rc >= 0) {
+        // Cmm gets the (partial) factor; R only on success, as the FP64 entry
+        if (hipMemcpyAsync(Cmm, dA, a_bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            (rc == 0 && b_elems &&
+             hipMemcpyAsync(Ctm, dB, b_elems * sizeof(float), hipMemcpyDeviceToHost, s) !=
+                 hipSuccess)) {
+            moao::set_error("sreconstructor: download failed");
+            rc = MOAO_ERR_HIP;
+        }
+    }
+    (void)hipFreeAsync(d, s);
+    if (hipStreamSynchronize(s) != hipSuccess && rc >= 0) rc = MOAO_ERR_HIP;
+    return rc;
 }
 
 }  // extern "C"

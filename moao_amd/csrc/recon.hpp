@@ -45,6 +45,11 @@ int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
 int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm, long ldCtm,
                           long ngal, double* dinvL, int* dinfo, hipStream_t stream);
 
+// The same on FP32 storage (FP32 MFMA updates, FP64 leaves); dinvL holds
+// recon_invl_doubles(n) floats.
+int reconstructor_f32_dev(long n, long m, float* dCmm, long ldCmm, float* dCtm, long ldCtm,
+                          long ngal, float* dinvL, int* dinfo, hipStream_t stream);
+
 // Number of elements of the inverse-leaf workspace.
 size_t recon_invl_doubles(long n);
 // Bytes of one FP64 solve's workspace: the inverse leaves, then (last 256 bytes) the status.

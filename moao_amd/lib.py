@@ -65,6 +65,17 @@ class ServeResult(ctypes.Structure):
                 ("t_start", ctypes.c_double), ("t_end", ctypes.c_double)]
 
 
+class IsampleStruct(ctypes.Structure):
+    """ctypes mirror of moao_isample_struct (the reference's isample_struct fields)."""
+    _fields_ = [("N", ctypes.c_long), ("np", ctypes.c_long), ("nidx", ctypes.c_long),
+                ("lambda_", ctypes.c_double), ("dactupix", ctypes.c_double),
+                ("idx", _c_long_p), ("abs2fi", _c_double_p), ("otf_tel", _c_double_p),
+                ("dphi", _c_double_p), ("filename_idx", ctypes.c_char * 256),
+                ("filename_abs2fi", ctypes.c_char * 256),
+                ("filename_otftel", ctypes.c_char * 256), ("precision", ctypes.c_int),
+                ("dev", ctypes.c_void_p)]
+
+
 class ServeCounters(ctypes.Structure):
     """ctypes mirror of moao_serve_counters."""
     _fields_ = [("queued", ctypes.c_long), ("running", ctypes.c_long),
@@ -118,6 +129,14 @@ _SIGS = {
     "moao_hip_reconstructor_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_void_p,
                                                   ctypes.c_long, ctypes.c_void_p, ctypes.c_long,
                                                   ctypes.c_long, ctypes.c_void_p]),
+    "moao_hip_sreconstructor": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
+                                               ctypes.POINTER(ctypes.c_float), ctypes.c_long,
+                                               ctypes.POINTER(ctypes.c_float), ctypes.c_long,
+                                               ctypes.c_long]),
+    "moao_hip_sreconstructor_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
+                                                   ctypes.c_void_p, ctypes.c_long,
+                                                   ctypes.c_void_p, ctypes.c_long, ctypes.c_long,
+                                                   ctypes.c_void_p]),
     "moao_hip_reconstructor_async": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
                                                     ctypes.c_void_p, ctypes.c_long,
                                                     ctypes.c_void_p, ctypes.c_long,
@@ -154,6 +173,18 @@ _SIGS = {
                                                         ctypes.c_int, ctypes.c_void_p,
                                                         ctypes.c_void_p]),
     "moao_hip_intersample_destroy": (None, [ctypes.c_void_p]),
+    "moao_hip_intersample_prepare": (None, [ctypes.POINTER(IsampleStruct), ctypes.c_long,
+                                            ctypes.c_long, ctypes.c_float, ctypes.c_char_p]),
+    "moao_hip_intersample_init": (ctypes.c_int, [ctypes.POINTER(IsampleStruct)]),
+    "moao_hip_intersample_process_d": (ctypes.c_int, [ctypes.POINTER(IsampleStruct),
+                                                      _c_double_p]),
+    "moao_hip_intersample_process_f": (ctypes.c_int, [ctypes.POINTER(IsampleStruct),
+                                                      ctypes.POINTER(ctypes.c_float)]),
+    "moao_hip_intersample_free": (None, [ctypes.POINTER(IsampleStruct)]),
+    "moao_fits_read": (ctypes.c_int, [ctypes.c_char_p, _c_long_p, _c_long_p, _c_double_p,
+                                      ctypes.c_long]),
+    "moao_fits_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_long, ctypes.c_long,
+                                       _c_double_p]),
     "moao_hip_profile_begin": (ctypes.c_int, []),
     "moao_hip_profile_end": (ctypes.c_int, [_c_double_p, _c_double_p, _c_long_p]),
     "moao_serve_create": (ctypes.c_int, [ctypes.POINTER(ServeConfig),

@@ -22,6 +22,18 @@ def reconstructor(nmeas: int, nmeasts: int, Cmm: np.ndarray, ldCmm: int, Ctm: np
                  "moao_hip_reconstructor")
 
 
+def sreconstructor(nmeas: int, nmeasts: int, Cmm: np.ndarray, ldCmm: int, Ctm: np.ndarray,
+                   ldCtm: int, ngal: int = 1) -> int:
+    """FP32 host drop-in (MOAO_CHAMELEON::reconstructor<float> arithmetic), in place."""
+    for a in (Cmm, Ctm):
+        if a.dtype != np.float32 or not (a.flags.f_contiguous or a.ndim == 1):
+            raise TypeError("Cmm and Ctm must be column-major float32")
+    fp = ctypes.POINTER(ctypes.c_float)
+    return check(lib().moao_hip_sreconstructor(nmeas, nmeasts, Cmm.ctypes.data_as(fp), ldCmm,
+                                               Ctm.ctypes.data_as(fp), ldCtm, ngal),
+                 "moao_hip_sreconstructor")
+
+
 def reconstructor_dev(nmeas: int, nmeasts: int, dCmm: int, ldCmm: int, dCtm: int, ldCtm: int,
                       ngal: int = 1, stream: int = 0) -> int:
     return check(lib().moao_hip_reconstructor_dev(nmeas, nmeasts, ctypes.c_void_p(dCmm), ldCmm,

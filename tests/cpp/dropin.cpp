@@ -7,8 +7,14 @@
 // (lapack/moao_lapack.cpp:108-121): matcov Cmm (type 1), Ctm (type 3), reconstructor.
 //
 //   dropin <sys-params> <atm-params> <R.bin>      writes R (m x n, column-major doubles)
+//   dropin --psf <dir> <nact> <np> <Cvv.bin> <psf.bin> [f32]
+//                                                 the reference's intersample sequence
+//                                                 (lapack/moao_lapack.cpp:57-66, 177-181) on
+//                                                 dir/idx.fits, abs2fi.fits, otftel.fits; Cvv
+//                                                 read as doubles (passed as float with f32)
 //   dropin --link-only                            checks the program links, no GPU use
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -40,6 +46,31 @@ static std::vector<double> field(moao_tomo* t, const char* name) {
 int main(int argc, char** argv) {
     if (argc == 2 && std::strcmp(argv[1], "--link-only") == 0) {
         std::printf("linked %s\n", moao_version());
+        return 0;
+    }
+    if (argc >= 7 && std::strcmp(argv[1], "--psf") == 0) {
+        const long nact = std::atol(argv[3]), np = std::atol(argv[4]);
+        std::vector<double> cvv((size_t)nact * nact);
+        FILE* f = std::fopen(argv[5], "rb");
+        if (!f || std::fread(cvv.data(), sizeof(double), cvv.size(), f) != cvv.size()) return 1;
+        std::fclose(f);
+        struct MOAO_HIP::isample_struct isample;
+        MOAO_HIP::intersample_prepare(&isample, nact * nact, np, 39.0f, argv[2]);
+        if (!MOAO_HIP::intersample_init(&isample)) return 1;
+        int ok;
+        if (argc == 8 && std::strcmp(argv[7], "f32") == 0) {
+            std::vector<float> c32(cvv.begin(), cvv.end());
+            ok = MOAO_HIP::intersample_process(&isample, c32.data());
+        } else {
+            ok = MOAO_HIP::intersample_process(&isample, cvv.data());
+        }
+        if (!ok) return 1;
+        FILE* g = std::fopen(argv[6], "wb");
+        if (!g) return 1;
+        std::fwrite(isample.dphi, sizeof(double), (size_t)isample.N * isample.N, g);
+        std::fclose(g);
+        std::printf("N=%ld dactupix=%g\n", isample.N, isample.dactupix);
+        MOAO_HIP::intersample_free(&isample);
         return 0;
     }
     if (argc != 4) {
