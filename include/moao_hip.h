@@ -31,6 +31,9 @@ extern "C" {
 /* Precision modes of the reconstructor. */
 #define MOAO_FP64 0            /* FP64 everywhere: the reference's lapack arithmetic */
 #define MOAO_MIXED_FP32 1      /* FP32 trailing updates, FP64 diagonal tiles + FP64 refinement */
+#define MOAO_MIXED_BF16 2      /* BF16 MFMA trailing updates (each FP32 operand split into two
+                                  BF16 terms, FP32 accumulation), FP64 diagonal tiles + FP64
+                                  refinement */
 
 const char* moao_last_error(void);
 const char* moao_version(void);
@@ -156,7 +159,8 @@ int moao_hip_sreconstructor_dev(long nmeas, long nmeasts, float* dCmm, long ldCm
  * exact lgs_cst low-rank term of the LGS blocks is removed before the FP32 factorization and
  * restored by the Woodbury identity in FP64.  Refinement stops when
  * ||Ctm - R Cmm||_F <= tol ||Ctm||_F (returned in *resid) or after max_iter corrections
- * (then MOAO_ERR_NOCONV with R holding the last iterate).  mode: MOAO_MIXED_FP32.           */
+ * (then MOAO_ERR_NOCONV with R holding the last iterate).  mode: MOAO_MIXED_FP32 or
+ * MOAO_MIXED_BF16.                                                                          */
 int moao_hip_reconstructor_mixed_dev(long nmeas, long nmeasts, const double* dCmm, long ldCmm,
                                      double* dCtm, long ldCtm, const moao_tomo_view* lowrank,
                                      int mode, double tol, int max_iter, double* resid,
@@ -247,6 +251,12 @@ int moao_hip_dgemm_dev(int transb, int M, int N, int K, double alpha, const doub
                        const double* B, long ldb, double beta, double* C, long ldc, void* stream);
 int moao_hip_sgemm_dev(int transb, int M, int N, int K, float alpha, const float* A, long lda,
                        const float* B, long ldb, float beta, float* C, long ldc, void* stream);
+/* The FP32 product on BF16 MFMA: each operand element split into two BF16 terms (hi =
+ * bf16(a), lo = bf16(a - hi)), hi*hi + hi*lo + lo*hi accumulated in FP32 (about 2^-16 relative
+ * per product).  transb 0 or 1.  The kernel of the mixed BF16 reconstructor's updates. */
+int moao_hip_sgemm_bf16x3_dev(int transb, int M, int N, int K, float alpha, const float* A,
+                              long lda, const float* B, long ldb, float beta, float* C, long ldc,
+                              void* stream);
 
 /* ---------------------------------------------------------------- timing ------------------
  * Per-kernel-class timing with HIP events recorded on each launch's stream.  No reference
@@ -279,7 +289,7 @@ typedef struct moao_serve_request {
     const char* sys_file;  /* reference-format sys-params file */
     const char* atm_file;  /* reference-format atmospheric profile */
     int target;            /* truth direction (Tomo_struct::setTarget) */
-    int mode;              /* MOAO_FP64 or MOAO_MIXED_FP32 */
+    int mode;              /* MOAO_FP64, MOAO_MIXED_FP32 or MOAO_MIXED_BF16 */
     double tol;            /* mixed: relative residual to refine to (0 -> 1e-11) */
     int max_iter;          /* mixed: refinement steps allowed (0 -> 5) */
     double* R;             /* host output m x n (column-major) or NULL; written before wait */

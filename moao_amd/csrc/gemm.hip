@@ -415,6 +415,165 @@ __global__ void __launch_bounds__(NT, TM == 128 ? 2 : 4) gemm_dl_kernel(GemmArgs
     }
 }
 
+// Split-BF16 kernel (FP32 storage).  Each operand element a is split as hi = bf16(a) and
+// lo = bf16(a - hi) (round to nearest even, v_cvt_pk_bf16_f32), and the product accumulates
+// lo*hi + hi*lo + hi*hi in FP32 on v_mfma_f32_16x16x32_bf16: about 16 significant bits per
+// operand (the dropped lo*lo term is 2^-16 relative) at three BF16 MFMAs per 16x16x32 step,
+// which together take 3/8 of the time of the eight FP32 MFMAs the same step costs.
+// The split runs once per element per tile, when the chunk is written to LDS: global loads of
+// chunk k+1 (FP32, registers) are issued before chunk k's MFMAs, then converted and stored as
+// hi / lo planes with k contiguous per row, so each MFMA operand (8 consecutive k of one row,
+// or of one column of op(B)) is a single 16-byte LDS read.  The MFMA pairs A's item i of lane
+// l with B's item i of the lane of the same l/16, so any common k order within a chunk gives
+// the same sum; the output layout is that of the FP32 16x16 MFMA (row 4*(l/16)+t, column l%16).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int KC = 32;      // k per chunk (one 16x16x32 step)
+constexpr int KP = KC + 8;  // LDS row pitch in BF16 elements (80 bytes)
+
+__device__ __forceinline__ void split8(const float* v, bf16x8& h, bf16x8& l) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const __bf16 x = (__bf16)v[i];
+        h[i] = x;
+        l[i] = (__bf16)(v[i] - (float)x);
+    }
+}
+
+template <int TB, int TM>
+__global__ void __launch_bounds__(NT, 2) gemm_split_kernel(GemmArgs<float> g) {
+    constexpr int WMB = TM / 32;            // 16-row MFMA blocks per wave along M
+    constexpr int KA = KC * TM / NT;        // A values each thread loads per chunk (16 or 8)
+    constexpr int KB = KC * BN / NT;        // B values each thread loads per chunk (16)
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[TM * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[TM * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[BN * KP];
+
+    if (g.skip_if && *(volatile const int*)g.skip_if != 0) return;
+    const GemmSeg<float> s = g.seg[blockIdx.z];
+    const int m0 = blockIdx.x * TM;
+    const int n0 = blockIdx.y * BN;
+    if (m0 >= s.M) return;
+    const bool tri = g.tri && blockIdx.z == 0;
+    if (tri && m0 + TM - 1 + g.tri_off < n0) return;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    const int K = g.K, N = g.N, M = s.M;
+    const int Kc = K > 0 ? K - 1 : 0;
+
+    f32x4 acc[WMB][4];
+#pragma unroll
+    for (int i = 0; i < WMB; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+
+    // A: thread -> row ra of the tile, k values [ka, ka + KA) of the chunk (lanes run down
+    // consecutive rows: every load instruction reads 64 consecutive floats of one column).
+    const int ra = tid % TM, ka = (tid / TM) * KA;
+    const bool ra_ok = m0 + ra < M;
+    const float* __restrict__ Ap = s.A + min(m0 + ra, M - 1);
+    // op(B) = B^T (B is N x K): as A.  op(B) = B (B is K x N): thread -> column rb, k values
+    // [kb, kb + KB) contiguous in memory.
+    const int rb = TB == 1 ? tid % BN : tid / 2;
+    const int kb = TB == 1 ? (tid / BN) * KB : (tid % 2) * KB;
+    const bool rb_ok = n0 + rb < N;
+    const float* __restrict__ Bp =
+        TB == 1 ? g.B + min(n0 + rb, N - 1) : g.B + (long)min(n0 + rb, N - 1) * g.ldb;
+    const long lda = s.lda, ldb = g.ldb;
+    float va[KA], vb[KB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < KA; ++r) {
+            const int k = k0 + ka + r;
+            const float x = Ap[(long)min(k, Kc) * lda];
+            va[r] = (ra_ok && k < K) ? x : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < KB; ++r) {
+            const int k = k0 + kb + r;
+            const float x = TB == 1 ? Bp[(long)min(k, Kc) * ldb] : Bp[min(k, Kc)];
+            vb[r] = (rb_ok && k < K) ? x : 0.0f;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int q = 0; q < KA / 8; ++q) {
+            bf16x8 h, l;
+            split8(va + 8 * q, h, l);
+            *(bf16x8*)&Ah[ra * KP + ka + 8 * q] = h;
+            *(bf16x8*)&Al[ra * KP + ka + 8 * q] = l;
+        }
+#pragma unroll
+        for (int q = 0; q < KB / 8; ++q) {
+            bf16x8 h, l;
+            split8(vb + 8 * q, h, l);
+            *(bf16x8*)&Bh[rb * KP + kb + 8 * q] = h;
+            *(bf16x8*)&Bl[rb * KP + kb + 8 * q] = l;
+        }
+    };
+
+    const int nk = (K + KC - 1) / KC;
+    const int ar = wm * 16 * WMB + (lane & 15);
+    const int bc = wn * 64 + (lane & 15);
+    const int kq = 8 * (lane >> 4);
+    if (nk > 0) {
+        load(0);
+        store();
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) load((kt + 1) * KC);
+        bf16x8 ah[WMB], al[WMB], bh[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < WMB; ++i) {
+            ah[i] = *(const bf16x8*)&Ah[(ar + 16 * i) * KP + kq];
+            al[i] = *(const bf16x8*)&Al[(ar + 16 * i) * KP + kq];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            bh[j] = *(const bf16x8*)&Bh[(bc + 16 * j) * KP + kq];
+            bl[j] = *(const bf16x8*)&Bl[(bc + 16 * j) * KP + kq];
+        }
+#pragma unroll
+        for (int i = 0; i < WMB; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+        if (kt + 1 < nk) {
+            __syncthreads();
+            store();
+            __syncthreads();
+        }
+    }
+
+    const float alpha = g.alpha, beta = g.beta;
+#pragma unroll
+    for (int i = 0; i < WMB; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+            if (col >= N) continue;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int row = m0 + wm * 16 * WMB + i * 16 + 4 * (lane >> 4) + t;
+                if (row >= M) continue;
+                if (tri && row + g.tri_off < col) continue;
+                float* c = s.C + row + (long)col * s.ldc;
+                float v = alpha * acc[i][j][t];
+                if (beta != 0.0f) v += beta * *c;
+                *c = v;
+            }
+        }
+    }
+}
+
 // out[b][r + 128c] = sym(B)[128b + r][128b + c] from the lower triangle of B.
 template <typename T>
 __global__ void symdiag_kernel(const T* __restrict__ B, long ldb, int K, T* __restrict__ out) {
@@ -455,6 +614,28 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
     ProfScope ps(sizeof(T) == 8 ? kProfGemmF64 : kProfGemmF32, work, stream);
     // 128-row tiles when they fill the device (>= 2 per CU), else 64-row tiles
     const bool small = (long)grid.x * grid.y * grid.z < 2L * 256;
+    if constexpr (std::is_same<T, float>::value) {
+        if (g.split) {
+            if (g.transb == 2) {
+                set_error("gemm: the split-BF16 product takes op(B) = B or B^T only");
+                return MOAO_ERR_ARG;
+            }
+            if (small) grid.x = (maxM + 63) / 64;
+            if (g.transb == 1) {
+                if (small)
+                    hipLaunchKernelGGL((gemm_split_kernel<1, 64>), grid, dim3(NT), 0, stream, g);
+                else
+                    hipLaunchKernelGGL((gemm_split_kernel<1, 128>), grid, dim3(NT), 0, stream, g);
+            } else {
+                if (small)
+                    hipLaunchKernelGGL((gemm_split_kernel<0, 64>), grid, dim3(NT), 0, stream, g);
+                else
+                    hipLaunchKernelGGL((gemm_split_kernel<0, 128>), grid, dim3(NT), 0, stream, g);
+            }
+            MOAO_HIP_TRY(hipGetLastError());
+            return MOAO_OK;
+        }
+    }
     // the K x N slab loads address 128 columns with 32-bit byte offsets
     const bool kn_offsets_fit = (long)BN * g.ldb * (long)sizeof(T) < (1L << 31);
     if (small) {
@@ -544,6 +725,22 @@ int moao_hip_sgemm_dev(int transb, int M, int N, int K, float alpha, const float
         return MOAO_ERR_ARG;
     }
     auto a = moao::gemm_args<float>(M, N, K, alpha, A, lda, B, ldb, transb, beta, C, ldc);
+    return moao::gemm_f32(a, moao::pick_stream(stream));
+}
+
+// The same product on the split-BF16 kernel (op(B) = B or B^T): what the mixed BF16
+// reconstructor's updates run.
+int moao_hip_sgemm_bf16x3_dev(int transb, int M, int N, int K, float alpha, const float* A,
+                              long lda, const float* B, long ldb, float beta, float* C, long ldc,
+                              void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldc < std::max(1, M) ||
+        ldb < std::max(1, transb == 1 ? N : K) || transb < 0 || transb > 1) {
+        moao::set_error("sgemm_bf16x3: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    auto a = moao::gemm_args<float>(M, N, K, alpha, A, lda, B, ldb, transb, beta, C, ldc);
+    a.split = 1;
     return moao::gemm_f32(a, moao::pick_stream(stream));
 }
 

@@ -44,6 +44,10 @@ struct GemmArgs {
     // transb = 2 only, optional: full symmetric copies of B's 128 x 128 diagonal blocks
     // (gemm_symdiag); with it the launch runs on the direct-to-LDS kernel.
     const T* symdiag;
+    // FP32 only: 1 = run the product on BF16 MFMA with each operand split into two BF16 terms
+    // (hi = bf16(a), lo = bf16(a - hi); hi*hi + hi*lo + lo*hi accumulated in FP32), op(B) = B or
+    // B^T; 0 = FP32 MFMA.
+    int split;
     // Optional device status word: when it is non-zero at launch time the whole launch is a
     // no-op.  The factorization points it at its dpotrf info, so the updates and solves queued
     // after a failing pivot do nothing and no host synchronisation is needed to stop them.
@@ -81,6 +85,7 @@ inline GemmArgs<T> gemm_args(int M, int N, int K, T alpha, const T* A, long lda,
     g.alpha = alpha;
     g.beta = beta;
     g.symdiag = nullptr;
+    g.split = 0;
     g.skip_if = nullptr;
     return g;
 }

@@ -206,14 +206,16 @@ size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank) {
 }
 
 int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, double* Ctm, long ldCtm,
-                            const moao_tomo_view* lowrank, double tol, int max_iter,
+                            const moao_tomo_view* lowrank, int mode, double tol, int max_iter,
                             double* resid_out, int* iters_out, void* ws, size_t ws_bytes,
                             hipStream_t s) {
     if (n < 1 || m < 1 || ldCmm < n || ldCtm < m || !Cmm || !Ctm || max_iter < 0 ||
-        n >= (1L << 31) || m >= (1L << 31)) {
+        n >= (1L << 31) || m >= (1L << 31) ||
+        (mode != MOAO_MIXED_FP32 && mode != MOAO_MIXED_BF16)) {
         set_error("reconstructor_mixed: invalid arguments");
         return MOAO_ERR_ARG;
     }
+    const int split = mode == MOAO_MIXED_BF16 ? 1 : 0;
     const bool debug = std::getenv("MOAO_DEBUG") != nullptr;
     const Ranges rg = lowrank_ranges(lowrank, n);
     const int p = rg.p();
@@ -288,7 +290,7 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
 
     // --- 2. FP32 factorization with [Ctm; U^T] riding along, second solve
     std::vector<RowGroup<float>> all{{w.X32, xr, (int)xr}};
-    if (int rc = potrf_tall<float>(n, w.C0, n, all, w.invL, w.info, s)) return rc;
+    if (int rc = potrf_tall<float>(n, w.C0, n, all, w.invL, w.info, s, split)) return rc;
     int hinfo = 0;
     MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
     MOAO_HIP_TRY(hipStreamSynchronize(s));
@@ -296,7 +298,8 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
         set_error("reconstructor_mixed: FP32 factorization of C0 failed");
         return hinfo;
     }
-    if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, all, s)) return rc;
+    if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, all, s, nullptr, split))
+        return rc;
 
     // --- 3. Woodbury terms
     std::vector<double> Sinv;
@@ -370,8 +373,14 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
         // D = E C0^{-1} (FP32 solves), Woodbury-corrected in FP64, R += D
         hipLaunchKernelGGL(cvt_f64_f32, grid2(M, N), dim3(256), 0, s, w.E, m, w.X32, xr, M, N);
         std::vector<RowGroup<float>> rhs{{w.X32, xr, M}};
-        if (int rc = trsm_right_lower_trans<float>(n, w.C0, n, w.invL, rhs, s)) { status = rc; break; }
-        if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, rhs, s)) { status = rc; break; }
+        if (int rc = trsm_right_lower_trans<float>(n, w.C0, n, w.invL, rhs, s, nullptr, split)) {
+            status = rc;
+            break;
+        }
+        if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, rhs, s, nullptr, split)) {
+            status = rc;
+            break;
+        }
         hipLaunchKernelGGL(cvt_f32_f64, grid2(M, N), dim3(256), 0, s, w.X32, xr, w.E, m, M, N);
         if (int rc = woodbury(w.E, m)) { status = rc; break; }
         hipLaunchKernelGGL(add_f64, grid2(M, N), dim3(256), 0, s, w.E, m, Ctm, ldCtm, M, N);
@@ -395,7 +404,7 @@ int moao_hip_reconstructor_mixed_dev(long nmeas, long nmeasts, const double* dCm
                                      int mode, double tol, int max_iter, double* resid,
                                      int* iters, void* stream) {
     if (int rc = moao::ensure_init()) return rc;
-    if (mode != MOAO_MIXED_FP32) {
+    if (mode != MOAO_MIXED_FP32 && mode != MOAO_MIXED_BF16) {
         moao::set_error("reconstructor_mixed: unsupported mode");
         return MOAO_ERR_ARG;
     }
@@ -413,8 +422,8 @@ int moao_hip_reconstructor_mixed_dev(long nmeas, long nmeasts, const double* dCm
                         " B failed");
         return MOAO_ERR_NOMEM;
     }
-    int rc = moao::reconstructor_mixed_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, lowrank, tol,
-                                           max_iter, resid, iters, ws, bytes, s);
+    int rc = moao::reconstructor_mixed_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, lowrank, mode,
+                                           tol, max_iter, resid, iters, ws, bytes, s);
     (void)hipFreeAsync(ws, s);
     if (hipStreamSynchronize(s) != hipSuccess && rc == MOAO_OK) {
         moao::set_error("reconstructor_mixed: stream synchronisation failed");

@@ -46,6 +46,7 @@ struct Tall {
     int* info;
     const int* skip;  // device status word: launches are no-ops once it is non-zero
     hipStream_t s;
+    int split;        // FP32 storage: update products on the split-BF16 kernel
     T* a(long r, long c) const { return A + r + c * lda; }
     T* inv(long j0) const { return invL + (j0 / kLeaf) * (long)kLeaf * kLeaf; }
 };
@@ -120,6 +121,7 @@ int potrf_rec(const Tall<T>& p, long j0, long nn) {
     proto.tri_off = 0;
     proto.alpha = T(-1);
     proto.beta = T(1);
+    proto.split = p.split;
     int rc = over_groups<T>(
         G, proto, true,
         [&] { return GemmSeg<T>{p.a(r0, j0), p.a(r0, r0), p.lda, p.lda, (int)(p.n - r0)}; },
@@ -165,6 +167,7 @@ int trsmN_rec(const Tall<T>& p, long j0, long nn) {
     proto.transb = 0;
     proto.alpha = T(-1);
     proto.beta = T(1);
+    proto.split = p.split;
     int rc = over_groups<T>(G, proto, false, none,
                             [&](const RowGroup<T>& g) {
                                 return GemmSeg<T>{g.p + (j0 + n1) * g.ld, g.p + j0 * g.ld, g.ld,
@@ -209,6 +212,7 @@ int trsmT_rec(const Tall<T>& p, long j0, long nn) {
     proto.transb = 1;
     proto.alpha = T(-1);
     proto.beta = T(1);
+    proto.split = p.split;
     int rc = over_groups<T>(G, proto, false, none,
                             [&](const RowGroup<T>& g) {
                                 return GemmSeg<T>{g.p + j0 * g.ld, g.p + (j0 + n1) * g.ld, g.ld,
@@ -225,8 +229,8 @@ __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 template <typename T>
 int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
-               int* info, hipStream_t s) {
-    Tall<T> p{n, A, lda, &groups, invL, info, info, s};
+               int* info, hipStream_t s, int split) {
+    Tall<T> p{n, A, lda, &groups, invL, info, info, s, split};
     // a kernel rather than a memset node, so the sequence can be captured into a graph
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, info, 0);
     MOAO_HIP_TRY(hipGetLastError());
@@ -236,35 +240,35 @@ int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T
 template <typename T>
 int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
                              const std::vector<RowGroup<T>>& groups, hipStream_t s,
-                             const int* skip) {
-    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s};
+                             const int* skip, int split) {
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s, split};
     return trsmN_rec(p, 0, n);
 }
 
 template <typename T>
 int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
                            const std::vector<RowGroup<T>>& groups, hipStream_t s,
-                           const int* skip) {
-    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s};
+                           const int* skip, int split) {
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s, split};
     return trsmT_rec(p, 0, n);
 }
 
 template int potrf_tall<double>(long, double*, long, const std::vector<RowGroup<double>>&,
-                                double*, int*, hipStream_t);
+                                double*, int*, hipStream_t, int);
 template int potrf_tall<float>(long, float*, long, const std::vector<RowGroup<float>>&, float*,
-                               int*, hipStream_t);
+                               int*, hipStream_t, int);
 template int trsm_right_lower_notrans<double>(long, const double*, long, const double*,
                                               const std::vector<RowGroup<double>>&, hipStream_t,
-                                              const int*);
+                                              const int*, int);
 template int trsm_right_lower_notrans<float>(long, const float*, long, const float*,
                                              const std::vector<RowGroup<float>>&, hipStream_t,
-                                             const int*);
+                                             const int*, int);
 template int trsm_right_lower_trans<double>(long, const double*, long, const double*,
                                             const std::vector<RowGroup<double>>&, hipStream_t,
-                                            const int*);
+                                            const int*, int);
 template int trsm_right_lower_trans<float>(long, const float*, long, const float*,
                                            const std::vector<RowGroup<float>>&, hipStream_t,
-                                           const int*);
+                                           const int*, int);
 
 int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm, long ldCtm,
                           long ngal, double* dinvL, int* dinfo, hipStream_t stream) {

@@ -21,20 +21,22 @@ struct RowGroup {
 // Recursive factorization of the n x n SPD block (lower, in place) with the row groups
 // solved by L^T on the way (first right-side solve fused).  invL receives the inverse of every
 // kLeaf leaf (ceil(n/kLeaf)*kLeaf*kLeaf elements); *info the dpotrf status (device int).
+// split (FP32 storage only): 1 runs the update products (not the leaves' panel products) on
+// the split-BF16 GEMM kernel.
 template <typename T>
 int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
-               int* info, hipStream_t s);
+               int* info, hipStream_t s, int split = 0);
 // Y * L = X for every group (second right-side solve), with L / invL from potrf_tall.
 // skip: optional device status word; every launch is a no-op once it is non-zero.
 template <typename T>
 int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
                              const std::vector<RowGroup<T>>& groups, hipStream_t s,
-                             const int* skip = nullptr);
+                             const int* skip = nullptr, int split = 0);
 // X * L^T = B for every group (first right-side solve, standalone).
 template <typename T>
 int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
                            const std::vector<RowGroup<T>>& groups, hipStream_t s,
-                           const int* skip = nullptr);
+                           const int* skip = nullptr, int split = 0);
 
 // FP64 path, stream-ordered (no host synchronisation).  dCmm (n x n, lower) is overwritten by
 // its Cholesky factor L; each of the ngal target blocks Ctm_g = dCtm + g*n*m (m x n, leading

@@ -163,7 +163,8 @@ int moao_server::run_body(moao::Request& r, moao::Slot& sl) {
         } else {
             double resid = 0;
             int iters = 0;
-            rc = reconstructor_mixed_dev(n, m, Cmm, n, Ctm, m, &r.view, r.tol, r.max_iter, &resid,
+            rc = reconstructor_mixed_dev(n, m, Cmm, n, Ctm, m, &r.view, r.mode, r.tol, r.max_iter,
+                                         &resid,
                                          &iters, scratch, r.solve_bytes, s);
             r.res.resid = resid;
             r.res.iters = iters;
@@ -270,7 +271,7 @@ int moao_serve_submit(moao_server* s, const moao_serve_request* q, long* ticket)
         moao::set_error("serve_submit: null argument");
         return MOAO_ERR_ARG;
     }
-    if (q->mode != MOAO_FP64 && q->mode != MOAO_MIXED_FP32) {
+    if (q->mode != MOAO_FP64 && q->mode != MOAO_MIXED_FP32 && q->mode != MOAO_MIXED_BF16) {
         moao::set_error("serve_submit: unknown mode");
         return MOAO_ERR_ARG;
     }

@@ -20,9 +20,10 @@ size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank);
 
 // Mixed-precision reconstructor (FP32 factorization of C0 = Cmm - lgs_cst U U^T, Woodbury,
 // FP64 refinement to tol).  Cmm is read only; Ctm is overwritten by R.  ws: device scratch of
-// at least mixed_workspace_bytes(n, m, lowrank) bytes, owned by the caller.
+// at least mixed_workspace_bytes(n, m, lowrank) bytes, owned by the caller.  mode:
+// MOAO_MIXED_FP32 (update products on FP32 MFMA) or MOAO_MIXED_BF16 (on the split-BF16 kernel).
 int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, double* Ctm, long ldCtm,
-                            const moao_tomo_view* lowrank, double tol, int max_iter,
+                            const moao_tomo_view* lowrank, int mode, double tol, int max_iter,
                             double* resid_out, int* iters_out, void* ws, size_t ws_bytes,
                             hipStream_t s);
 

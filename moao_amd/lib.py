@@ -21,6 +21,7 @@ MOAO_ERR_NODEV = -6
 
 MOAO_FP64 = 0
 MOAO_MIXED_FP32 = 1
+MOAO_MIXED_BF16 = 2
 
 _c_long_p = ctypes.POINTER(ctypes.c_long)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -158,6 +159,11 @@ _SIGS = {
                                           ctypes.c_float, ctypes.c_void_p, ctypes.c_long,
                                           ctypes.c_void_p, ctypes.c_long, ctypes.c_float,
                                           ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p]),
+    "moao_hip_sgemm_bf16x3_dev": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                                                 ctypes.c_long, ctypes.c_void_p, ctypes.c_long,
+                                                 ctypes.c_float, ctypes.c_void_p, ctypes.c_long,
+                                                 ctypes.c_void_p]),
     "moao_hip_cee_cvv": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_long]
                          + [_c_double_p, ctypes.c_long] * 8),
     "moao_hip_cee_cvv_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_long]

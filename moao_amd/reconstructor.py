@@ -55,16 +55,17 @@ def reconstructor_async(nmeas: int, nmeasts: int, dCmm: int, ldCmm: int, dCtm: i
 
 def reconstructor_mixed_dev(nmeas: int, nmeasts: int, dCmm: int, ldCmm: int, dCtm: int,
                             ldCtm: int, tomo=None, tol: float = 1e-11, max_iter: int = 5,
-                            stream: int = 0, allow_noconv: bool = False):
-    """Mixed-precision solve (FP32 factorization + FP64 refinement), Ctm -> R in place, Cmm
-    preserved.  Returns (status, relative residual, refinement steps)."""
+                            stream: int = 0, allow_noconv: bool = False, mode=None):
+    """Mixed-precision solve (FP32 or split-BF16 factorization + FP64 refinement), Ctm -> R in
+    place, Cmm preserved.  Returns (status, relative residual, refinement steps)."""
     from .lib import MOAO_ERR_NOCONV, MOAO_MIXED_FP32
+    mode = MOAO_MIXED_FP32 if mode is None else mode
     view = tomo.view() if tomo is not None else None
     res = ctypes.c_double()
     its = ctypes.c_int()
     rc = lib().moao_hip_reconstructor_mixed_dev(
         nmeas, nmeasts, ctypes.c_void_p(dCmm), ldCmm, ctypes.c_void_p(dCtm), ldCtm,
-        ctypes.byref(view) if view is not None else None, MOAO_MIXED_FP32, tol, max_iter,
+        ctypes.byref(view) if view is not None else None, mode, tol, max_iter,
         ctypes.byref(res), ctypes.byref(its), ctypes.c_void_p(stream))
     if not (allow_noconv and rc == MOAO_ERR_NOCONV):
         check(rc, "moao_hip_reconstructor_mixed_dev")
@@ -81,3 +82,10 @@ def sgemm_dev(transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, stream=0)
     check(lib().moao_hip_sgemm_dev(transb, M, N, K, alpha, ctypes.c_void_p(dA), lda,
                                    ctypes.c_void_p(dB), ldb, beta, ctypes.c_void_p(dC), ldc,
                                    ctypes.c_void_p(stream)), "moao_hip_sgemm_dev")
+
+
+def sgemm_bf16x3_dev(transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, stream=0):
+    check(lib().moao_hip_sgemm_bf16x3_dev(transb, M, N, K, alpha, ctypes.c_void_p(dA), lda,
+                                          ctypes.c_void_p(dB), ldb, beta, ctypes.c_void_p(dC),
+                                          ldc, ctypes.c_void_p(stream)),
+          "moao_hip_sgemm_bf16x3_dev")

@@ -9,7 +9,9 @@
     the requests with n <= 10k.
 
 Tolerances: FP64 R within 1e-9 of the oracle at n = 39,720 (kappa(Cmm) ~ 1e8 there, so two FP64
-solvers differ at the 1e-10 level; backward error <= 5e-13 is what a stable solver guarantees);
+solvers differ at the 1e-10 level); backward error within 3x the oracle's own (a stable solver's
+backward error grows like n u ||R|| ||Cmm|| / ||Ctm||: LAPACK's is 7.4e-13 at n = 39,720), and
+3e-12 for the stream's requests above n = 10k, which get no oracle solve;
 mixed R and Cee within 1e-6 (the mixed path's specification).
 """
 import ctypes
@@ -98,7 +100,7 @@ def test_elt74_fp64_and_mixed_against_oracle(tmp_path):
     bwdo = _backward_error(dC0, dB0, Ro, n, m)
     print(f"    FP64: |R - R_lapack|/|R_lapack| = {err64:.2e}, backward {bwd64:.2e} "
           f"(oracle's own {bwdo:.2e})", flush=True)
-    assert bwd64 <= 5e-13
+    assert bwd64 <= max(5e-13, 3.0 * bwdo)
     assert err64 <= 1e-9
     print("[elt74] mixed FP32 solve (refined to 1e-10)", flush=True)
     dB.copy_from(dB0)
@@ -148,14 +150,19 @@ def test_serving_stream_of_uneven_requests(tmp_path):
         dC, dB = _generate(t)
         bwd = _backward_error(dC, dB, mix_R[k], n, m)
         line = f"    request {k} n={n} m={m} mode={md}: backward {bwd:.2e}"
-        assert bwd <= (5e-13 if md == MOAO_FP64 else 2e-11)
+        bar = 3e-12 if md == MOAO_FP64 else 2e-11
         if k in keep:
             Ro = dB.to_host()
             assert lp.reconstructor(dC.to_host(), Ro) == 0
             err = rel_fro(mix_R[k], Ro)
-            line += f", vs oracle {err:.2e}"
-            assert err <= (1e-10 if md == MOAO_FP64 else 1e-6)
+            bwdo = _backward_error(dC, dB, Ro, n, m)
+            line += f" (oracle's own {bwdo:.2e}), vs oracle {err:.2e}"
+            if md == MOAO_FP64:
+                bar = max(5e-13, 3.0 * bwdo)
         print(line, flush=True)
+        assert bwd <= bar
+        if k in keep:
+            assert err <= (1e-10 if md == MOAO_FP64 else 1e-6)
         dC.free()
         dB.free()
         del mix_R[k]

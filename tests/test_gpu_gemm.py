@@ -89,3 +89,45 @@ def test_gemm_reads_only_the_operands(M, N, K, transb):
     assert np.array_equal(outs[0], outs[1])
     ref = 0.5 * A[:M] @ op_b(B, transb, N, K) + C
     assert np.allclose(outs[0], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+SPLIT_CASES = [(128, 128, 32, 1, 0.0), (130, 77, 33, 1, 1.0), (300, 129, 200, 0, -1.0),
+               (1, 1, 1, 0, 0.5), (257, 383, 700, 1, 1.0), (64, 500, 5, 0, 0.0),
+               (4133, 3900, 300, 1, 1.0), (4133, 3900, 301, 0, -1.0), (4133, 2100, 3001, 1, 1.0)]
+
+
+@pytest.mark.parametrize("M,N,K,transb,beta", SPLIT_CASES)
+def test_split_bf16_gemm_matches_numpy(M, N, K, transb, beta):
+    """The split-BF16 product (hi*hi + hi*lo + lo*hi on BF16 MFMA, FP32 accumulation): each
+    product carries a relative error of at most ~3 * 2^-16 (the dropped lo*lo term and the
+    residue of the two-term split), so |C - C_ref| <= 1e-4 (|A||op(B)| + |beta C|) elementwise
+    plus the FP32 accumulation bound; and the relative Frobenius error of the product stays
+    below 1e-4, where one BF16 term per operand would give ~2e-3.  NaN planted in the
+    leading-dimension padding must not reach C."""
+    from moao_amd.reconstructor import sgemm_bf16x3_dev
+    print(f"[gemm-bf16x3] M={M} N={N} K={K} transb={transb} beta={beta}", flush=True)
+    rng = np.random.default_rng(M * 5 + N * 3 + K)
+    lda, ldb, ldc = M + 3, (N if transb == 1 else K) + 5, M + 1
+    A = np.asfortranarray(rng.standard_normal((lda, K)).astype(np.float32))
+    B = np.asfortranarray(rng.standard_normal((ldb, N if transb == 0 else K)).astype(np.float32))
+    C = np.asfortranarray(rng.standard_normal((ldc, N)).astype(np.float32))
+    A[M:] = np.nan
+    B[(N if transb == 1 else K):] = np.nan
+    alpha = -0.75
+    dA, dB, dC = DeviceArray.from_host(A), DeviceArray.from_host(B), DeviceArray.from_host(C)
+    sgemm_bf16x3_dev(transb, M, N, K, alpha, dA.ptr, lda, dB.ptr, ldb, beta, dC.ptr, ldc)
+    out = dC.to_host()
+    a = A[:M].astype(np.float64)
+    b = op_b(B, transb, N, K)
+    prod = alpha * a @ b
+    ref = prod + beta * C[:M].astype(np.float64)
+    scale = np.abs(alpha) * np.abs(a) @ np.abs(b) + abs(beta) * np.abs(C[:M])
+    tol = 1e-4 + 2e-5 * max(1.0, K / 700.0)
+    err = np.abs(out[:M] - ref)
+    assert np.all(np.isfinite(out[:M]))
+    assert np.all(err <= tol * (scale + 1e-30)), float(np.max(err / (scale + 1e-30)))
+    rel = np.linalg.norm(out[:M] - ref) / np.linalg.norm(prod)
+    print(f"    max normalised error {np.max(err / (scale + 1e-30)):.2e}, "
+          f"relative Frobenius {rel:.2e}", flush=True)
+    assert rel <= 1e-4
+    assert np.array_equal(out[M:], C[M:])
