@@ -17,7 +17,11 @@
 //    a host-built int32 table per row and per column;
 //  * per-layer constants 2*pi/L0 and k1*L0^(5/3) are computed once on the host with the
 //    reference's own expressions (identical bits), not per call of the structure function;
-//  * type 1 skips workgroups whose tile lies wholly above the diagonal.
+//  * type 1 skips workgroups whose tile lies wholly above the diagonal;
+//  * a whole Cmm or Ctm (the lapack path's call) runs pair_kernel instead: one lane per pair
+//    of subapertures computes the pair's xx, xy, yx and yy elements, sharing the structure
+//    functions those four have in common (9 evaluations per layer instead of 14 for sensors
+//    of equal subaperture size), bit-identical to the element kernel.
 // Contraction into FMA is disabled here so each product/sum rounds as in the reference; the
 // only differences left are the last-ulp differences of the device pow/exp/sqrt.
 #pragma clang fp contract(off)
@@ -86,6 +90,13 @@ struct MatcovArgs {
     const double* Y;
     int truth_base;         // Nsubap_t*(Nw-1), index of the truth sensor's first subap
     int truth_nsub;         // Nsubap of the truth sensor
+    // pair kernel (whole matrices of types 1 and 3): one entry per subaperture of the row and
+    // column sensors, {(pos << 9) | (wfs << 1), matrix index of its x slope}; the y slope sits
+    // nsub[wfs] further
+    const int2* rsub;
+    const int2* csub;
+    const int* nsub;        // [Nw]
+    int nrsub, ncsub;
 };
 
 // cx = cbrt(x), shared with vk_asymptote: lanes of one wave often take both branches.
@@ -198,6 +209,97 @@ __global__ void __launch_bounds__(kWaves * 64) meas_kernel(MatcovArgs a) {
     }
 }
 
+// Whole Cmm (type 1) or Ctm (type 3): one lane per pair of subapertures computes the 2 x 2
+// block xx, xy, yx, yy of that pair, so the four slope covariances share their structure
+// functions.  Per layer, for equal subaperture sizes (every pair of sensors at one altitude),
+// xx and yy share D(du, dv), and xy and yx use the same four corners (du +- s, dv +- s): 9
+// evaluations instead of the 14 the four elements take one by one.  Every element keeps the
+// element kernel's expression and order (same arguments, same sums, same layer order), so the
+// result is bit-identical to it.  Lanes run down consecutive row subapertures: the x and y
+// rows of a pair are two contiguous runs of each column.
+__global__ void __launch_bounds__(kWaves * 64) pair_kernel(MatcovArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int rs = blockIdx.x * kRows + lane;
+    const int cs0 = blockIdx.y * kCols + wave * kColsPerWave;
+    if (rs >= a.nrsub) return;
+    const int2 re = a.rsub[rs];
+    const int m = (re.x >> 1) & 0xff, p1 = re.x >> 9;
+    const int xr = re.y, yr = re.y + a.nsub[m];
+    for (int c = 0; c < kColsPerWave; ++c) {
+        const int cs = cs0 + c;
+        if (cs >= a.ncsub) break;
+        const int2 ce = a.csub[cs];
+        const int n = (ce.x >> 1) & 0xff, p2 = ce.x >> 9;
+        const int xc = ce.y, yc = ce.y + a.nsub[n];
+        // type 1 writes only row >= column: nothing of this pair when its largest row is above
+        // its smallest column
+        if (a.type_mat == 1 && yr < xc) continue;
+        const double kk = kLambda2 / (a.sspsz[m] * a.sspsz[n]);
+        double cxx = 0.0, cxy = 0.0, cyx = 0.0, cyy = 0.0;
+        for (int l = 0; l < a.Nlayer; ++l) {
+            const double sml = a.sspL[m * a.Nlayer + l];
+            const double snl = a.sspL[n * a.Nlayer + l];
+            if ((sml > 0) && (snl > 0)) {
+                const long o = (long)l * a.Nx;
+                const double du = a.u[o + p1] - a.u[o + p2];
+                const double dv = a.v[o + p1] - a.v[o + p2];
+                const double s1 = sml * 0.5, s2 = snl * 0.5;
+                const double lk = a.lk[l], amp = a.amp[l];
+                const double ac = s1 - s2, ad = s1 + s2, bc = -ad, bd = -ac;
+                double sxx, syy, sxy, syx;
+                if (ac == 0.0) {
+                    const double d0 = vk_dphi(du, dv, lk, amp);
+                    sxx = -d0 + vk_dphi(du + ad, dv, lk, amp) + vk_dphi(du + bc, dv, lk, amp) - d0;
+                    syy = -d0 + vk_dphi(du, dv + ad, lk, amp) + vk_dphi(du, dv + bc, lk, amp) - d0;
+                    // s1 == s2: the corners of xy and yx are the same four arguments
+                    const double q1 = vk_dphi(du + s1, dv - s2, lk, amp);
+                    const double q2 = vk_dphi(du + s1, dv + s2, lk, amp);
+                    const double q3 = vk_dphi(du - s1, dv - s2, lk, amp);
+                    const double q4 = vk_dphi(du - s1, dv + s2, lk, amp);
+                    sxy = -q1 + q2 + q3 - q4;
+                    syx = -q4 + q2 + q3 - q1;
+                } else {
+                    sxx = -vk_dphi(du + ac, dv, lk, amp) + vk_dphi(du + ad, dv, lk, amp) +
+                          vk_dphi(du + bc, dv, lk, amp) - vk_dphi(du + bd, dv, lk, amp);
+                    syy = -vk_dphi(du, dv + ac, lk, amp) + vk_dphi(du, dv + ad, lk, amp) +
+                          vk_dphi(du, dv + bc, lk, amp) - vk_dphi(du, dv + bd, lk, amp);
+                    sxy = -vk_dphi(du + s1, dv - s2, lk, amp) + vk_dphi(du + s1, dv + s2, lk, amp) +
+                          vk_dphi(du - s1, dv - s2, lk, amp) - vk_dphi(du - s1, dv + s2, lk, amp);
+                    syx = -vk_dphi(du - s2, dv + s1, lk, amp) + vk_dphi(du + s2, dv + s1, lk, amp) +
+                          vk_dphi(du - s2, dv - s1, lk, amp) - vk_dphi(du + s2, dv - s1, lk, amp);
+                }
+                cxx += 0.5 * sxx * kk * a.cn2[l];
+                cxy += 0.5 * sxy * kk * a.cn2[l];
+                cyx += 0.5 * syx * kk * a.cn2[l];
+                cyy += 0.5 * syy * kk * a.cn2[l];
+            }
+        }
+        if (m == n) {
+            if (m < a.nlgs) {
+                if (p1 == p2) {
+                    cxx += a.nxx[p1];
+                    cyy += a.nyy[p1];
+                    cxy += a.nxy[p1];
+                    cyx += a.nxy[p1];
+                }
+                cxx += a.lgs_cst;
+                cyy += a.lgs_cst;
+            } else if (p1 == p2) {
+                cxx += a.nNGS[m - a.nlgs];
+                cyy += a.nNGS[m - a.nlgs];
+            }
+        }
+        const bool lo = a.type_mat == 1;
+        double* ox = a.out + (long)xc * a.lda;
+        double* oy = a.out + (long)yc * a.lda;
+        if (!lo || xr >= xc) ox[xr] = cxx;
+        if (!lo || yr >= xc) ox[yr] = cyx;
+        if (!lo || xr >= yc) oy[xr] = cxy;
+        if (!lo || yr >= yc) oy[yr] = cyy;
+    }
+}
+
 __global__ void __launch_bounds__(kWaves * 64) truth_kernel(MatcovArgs a) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -294,6 +396,25 @@ int matcov_tile_launch(double* d_data, int nrows, int ncols, int xoffset, int yo
             coltab[c] = (type_mat == 2) ? pack_truth(c + yoffset, t, off)
                                         : pack_meas(c + yoffset, t, off);
     }
+    // Whole Cmm / Ctm: the pair kernel (one lane per subaperture pair).
+    const bool pairs = (type_mat == 1 || type_mat == 3) && xoffset == 0 && yoffset == 0 &&
+                       nrows == row_lim && ncols == col_lim;
+    std::vector<int2> rsub, csub;
+    std::vector<int> nsub(Nw);
+    for (int w = 0; w < Nw; ++w) nsub[w] = (int)t->Nsubap[w];
+    if (pairs) {
+        for (int w = 0; w < Nw - 1; ++w)
+            for (long i = 0; i < t->Nsubap[w]; ++i)
+                csub.push_back(make_int2((int)(((off[w] + i) << 9) | (w << 1)),
+                                         (int)(2 * off[w] + i)));
+        if (type_mat == 1) {
+            rsub = csub;
+        } else {
+            const int w = Nw - 1;
+            for (long i = 0; i < t->Nsubap[w]; ++i)
+                rsub.push_back(make_int2((int)(((off[w] + i) << 9) | (w << 1)), (int)i));
+        }
+    }
     std::vector<double> lk(L), amp(L), sspsz(Nw);
     for (int l = 0; l < L; ++l) {
         const double L0 = t->L0[l];
@@ -328,6 +449,9 @@ int matcov_tile_launch(double* d_data, int nrows, int ncols, int xoffset, int yo
     const size_t iXY = add(t->noiseLGSxy, sizeof(double) * Nx);
     const size_t iX = add(t->X, sizeof(double) * Nx);
     const size_t iY = add(t->Y, sizeof(double) * Nx);
+    const size_t iRs = add(rsub.data(), sizeof(int2) * rsub.size());
+    const size_t iCs = add(csub.data(), sizeof(int2) * csub.size());
+    const size_t iNs = add(nsub.data(), sizeof(int) * Nw);
 
     std::vector<unsigned char> host(total);
     for (auto& p : pieces)
@@ -375,6 +499,11 @@ int matcov_tile_launch(double* d_data, int nrows, int ncols, int xoffset, int yo
     a.Y = (const double*)P(iY);
     a.truth_nsub = (int)t->Nsubap[Nw - 1];
     a.truth_base = (int)(t->Nsubap[Nw - 1] * (Nw - 1));
+    a.rsub = (const int2*)P(iRs);
+    a.csub = (const int2*)P(iCs);
+    a.nsub = (const int*)P(iNs);
+    a.nrsub = (int)rsub.size();
+    a.ncsub = (int)csub.size();
 
     dim3 grid((nrows + kRows - 1) / kRows, (ncols + kCols - 1) / kCols);
     if (grid.y > 65535) {
@@ -390,7 +519,10 @@ int matcov_tile_launch(double* d_data, int nrows, int ncols, int xoffset, int yo
                 elems += std::max(0, std::min(nrows, xoffset + nrows - (yoffset + c)));
         }
         ProfScope ps(kProfMatcov, elems, stream);
-        if (type_mat == 4)
+        if (pairs) {
+            const dim3 pg((a.nrsub + kRows - 1) / kRows, (a.ncsub + kCols - 1) / kCols);
+            hipLaunchKernelGGL(pair_kernel, pg, dim3(kWaves * 64), 0, stream, a);
+        } else if (type_mat == 4)
             hipLaunchKernelGGL(truth_kernel, grid, dim3(kWaves * 64), 0, stream, a);
         else
             hipLaunchKernelGGL(meas_kernel, grid, dim3(kWaves * 64), 0, stream, a);

@@ -123,3 +123,29 @@ def test_elt_workload_tiles(tmp_path):
         err = np.max(np.abs(g - r)) / scale[typ]
         print(f"    type {typ} tile at ({xo}, {yo}): {err:.2e} of the diagonal scale", flush=True)
         assert err <= 1e-12
+
+
+def test_whole_matrix_pair_kernel_is_bit_identical_to_element_tiles(tmp_path):
+    """A whole Cmm or Ctm runs the pair kernel (one lane per subaperture pair, shared structure
+    functions); any other tile runs the element kernel.  Both must give the same bits: the
+    fixtures (LGS only, equal subaperture sizes per layer) and the n = 18,688 system (LGS and
+    NGS: unequal sizes on the layers the LGS cone shrinks), column blocks of the whole matrix
+    against the same columns generated as tiles at a column offset."""
+    from moao_amd import systems
+    s, a = systems.make_system(str(tmp_path), "elt")
+    cases = [Tomo(*fixture_system(FIXTURES[0])), Tomo(*fixture_system(FIXTURES[1])), Tomo(s, a[0])]
+    for t in cases:
+        n, m = t.nmeas(), t.nmeasts()
+        t.set_target(0)
+        print(f"[matcov-pairs] n={n} m={m}", flush=True)
+        C = gm.cmm(t)
+        B = gm.ctm(t)
+        W = 300
+        for c0 in sorted({0, n // 3, n - W}):
+            blk = np.full((n, W), -1.0, order="F")
+            gm.matcov_tile(blk, n, W, 0, c0, n, t, gm.CMM)
+            lower = np.arange(n)[:, None] >= np.arange(c0, c0 + W)[None, :]
+            assert np.array_equal(blk[lower], C[:, c0:c0 + W][lower]), (n, "Cmm", c0)
+            bb = np.empty((m, W), order="F")
+            gm.matcov_tile(bb, m, W, 0, c0, m, t, gm.CTM)
+            assert np.array_equal(bb, B[:, c0:c0 + W]), (n, "Ctm", c0)

@@ -152,14 +152,22 @@ def test_overlapping_targets_are_rejected():
     assert e.value.code == MOAO_ERR_ARG
 
 
-def _partial_factor_ok(Cg, S, ncols):
-    """The first ncols columns of the GPU factor equal the Cholesky factor's columns, computed
-    from the leading ncols x ncols block (numpy) and the rows below by a triangular solve."""
+def _partial_factor_ok(Cg, S, ncols, leaf0):
+    """After a failing pivot in the leaf starting at column leaf0, the factor is complete in
+    every column of the earlier leaves (all rows) and, in the failing leaf's 16-column blocks
+    before the failing block (columns leaf0..ncols-1), in the leaf's own rows: the panel rows
+    below a leaf are solved only once the whole leaf is factored.  Reference columns: the
+    Cholesky factor of the leading block (numpy) and the rows below by a triangular solve."""
     L11 = np.linalg.cholesky(S[:ncols, :ncols])
     L21 = np.linalg.solve(L11, S[ncols:, :ncols].T).T
     ref = np.vstack([L11, L21])
     got = np.tril(Cg[:, :ncols])
-    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-12
+    full = slice(0, leaf0)
+    assert np.linalg.norm(got[:, full] - ref[:, full]) / np.linalg.norm(ref[:, full]) <= 1e-12
+    rows = slice(leaf0, min(leaf0 + 128, S.shape[0]))
+    part = slice(leaf0, ncols)
+    assert (np.linalg.norm(got[rows, part] - ref[rows, part]) /
+            np.linalg.norm(ref[rows, part]) <= 1e-12)
 
 
 def test_two_streams_overlap():
@@ -210,10 +218,10 @@ def test_not_positive_definite_reports_info():
     info = reconstructor(n, m, Cg, n, Bg, m, 1)
     assert info == info_ref == 301
     assert np.array_equal(Bg, B)  # R not computed: Ctm untouched on the host entry
-    # column 300 is local column 44 of the leaf [256, 384): the leaves before it and that
-    # leaf's 16-column blocks before column 288 hold the factor
+    # column 300 is local column 44 of the leaf [256, 384): the leaves before it and, in that
+    # leaf's rows, its 16-column blocks before column 288 hold the factor
     S = np.tril(C) + np.tril(C, -1).T
-    _partial_factor_ok(Cg, S, 288)
+    _partial_factor_ok(Cg, S, 288, 256)
 
 
 def test_nan_pivot_reports_info():
@@ -233,6 +241,9 @@ def test_nan_pivot_reports_info():
     Cg = C.copy(order="F")
     assert reconstructor(n, m, Cg, n, Bg, m, 1) == 201
     assert np.array_equal(Bg, B)
-    # leaf [128, 256): its blocks before column 192 (the one holding column 200) are stored
+    # leaf [128, 256): its blocks before column 192 (the one holding column 200) are stored in
+    # the leaf's rows.  The NaN sits in the block [144, 160), whose rows are formed as one
+    # product with the block's inverse (0 * NaN = NaN), so that block's row 200 is NaN in
+    # every column; the blocks before it are checked.
     S = np.tril(C) + np.tril(C, -1).T
-    _partial_factor_ok(Cg, S, 150)
+    _partial_factor_ok(Cg, S, 144, 128)
