@@ -142,6 +142,20 @@ int moao_hip_reconstructor_dev(long nmeas, long nmeasts, double* dCmm, long ldCm
 int moao_hip_reconstructor_async(long nmeas, long nmeasts, double* dCmm, long ldCmm,
                                  double* dCtm, long ldCtm, long ngal, int* d_info, void* stream);
 
+/* The same solve as moao_hip_reconstructor_async with no allocation of any kind: its workspace
+ * (the inverse leaves) is a static device array of 40 MiB that each device holds for the life
+ * of the process, so nmeas is limited to 40960 (larger orders return MOAO_ERR_ARG).  *d_info
+ * is used as the solve's status word and holds the dpotrf status when the solve has run.  No
+ * counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, fixed serving loops).  Every call on one device shares the array, so
+ * calls that overlap on one device (different streams, or threads) race; calls ordered on one
+ * stream do not.  Call moao_hip_init first when the caller must not create anything: the
+ * library's init takes the scratch address (and creates the library stream used when stream
+ * is NULL). */
+int moao_hip_reconstructor_noalloc(long nmeas, long nmeasts, double* dCmm, long ldCmm,
+                                   double* dCtm, long ldCtm, long ngal, int* d_info,
+                                   void* stream);
+
 /* The same solve on FP32 storage: MOAO_CHAMELEON::reconstructor<float>, the precision the
  * reference's Chameleon driver runs in production (chameleon/moao_chameleon.cpp:41,
  * chameleon/compute/reconstructor.cpp:31-65: spotrf + 2 strsm).  FP32 MFMA updates, the

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -319,6 +321,48 @@ size_t recon_workspace_bytes(long n) {
     return ((recon_invl_doubles(n) * sizeof(double) + 255) & ~size_t(255)) + 256;
 }
 
+// ---- scratch of the allocation-free entry -------------------------------------------------
+// The inverse leaves of one FP64 solve of order up to kNoAllocMaxN live in this array (40 MiB),
+// which every device the library runs on holds until the process ends.  In host code the name
+// `scratch` is only a host-side stand-in: kernels get the per-device address that
+// hipGetSymbolAddress returns (taken once per device with that device current, at init or on
+// the entry's first call there), never the symbol itself.
+constexpr long kNoAllocMaxN = 40960;
+constexpr size_t kNoAllocElems = (size_t)(kNoAllocMaxN / kLeaf) * kLeaf * kLeaf;
+static __device__ __align__(256) double scratch[kNoAllocElems];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch_setup() {
+    int dev = -1;
+    MOAO_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("allocation-free scratch: device index out of range");
+        return MOAO_ERR_NODEV;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch[dev]) return MOAO_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return MOAO_ERR_HIP;
+    }
+    g_scratch[dev] = static_cast<double*>(p);
+    return MOAO_OK;
+}
+
+static double* noalloc_scratch_ptr() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return g_scratch[dev];
+}
+
 }  // namespace moao
 
 extern "C" {
@@ -352,6 +396,34 @@ int moao_hip_reconstructor_async(long nmeas, long nmeasts, double* dCmm, long ld
     }
     (void)hipFreeAsync(ws, s);
     return rc;
+}
+
+// The stream-ordered solve with its workspace in the static device scratch: no allocation,
+// so callers that cannot allocate (graph capture, fixed serving loops) can use it.  *d_info is
+// the solve's status word itself (cleared by a kernel at the start).
+int moao_hip_reconstructor_noalloc(long nmeas, long nmeasts, double* dCmm, long ldCmm,
+                                   double* dCtm, long ldCtm, long ngal, int* d_info,
+                                   void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (!d_info || nmeas < 1) {
+        moao::set_error("reconstructor_noalloc: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    if (nmeas > moao::kNoAllocMaxN) {
+        moao::set_error("reconstructor_noalloc: nmeas = " + std::to_string(nmeas) +
+                        " exceeds the scratch maximum of " + std::to_string(moao::kNoAllocMaxN) +
+                        " (use moao_hip_reconstructor_async)");
+        return MOAO_ERR_ARG;
+    }
+    // the device's scratch address (taken at init; here on the first call on another device)
+    if (int rc = moao::noalloc_scratch_setup()) return rc;
+    double* invL = moao::noalloc_scratch_ptr();
+    if (!invL) {
+        moao::set_error("reconstructor_noalloc: no scratch address for the current device");
+        return MOAO_ERR_HIP;
+    }
+    return moao::reconstructor_f64_dev(nmeas, nmeasts, dCmm, ldCmm, dCtm, ldCtm, ngal, invL,
+                                       d_info, moao::pick_stream(stream));
 }
 
 int moao_hip_reconstructor_dev(long nmeas, long nmeasts, double* dCmm, long ldCmm, double* dCtm,

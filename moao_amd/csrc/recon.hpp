@@ -52,6 +52,10 @@ int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm
 int reconstructor_f32_dev(long n, long m, float* dCmm, long ldCmm, float* dCtm, long ldCtm,
                           long ngal, float* dinvL, int* dinfo, hipStream_t stream);
 
+// Takes the current device's address of the allocation-free entry's static scratch (once per
+// device, with that device current); called by the library's init.
+int noalloc_scratch_setup();
+
 // Number of elements of the inverse-leaf workspace.
 size_t recon_invl_doubles(long n);
 // Bytes of one FP64 solve's workspace: the inverse leaves, then (last 256 bytes) the status.

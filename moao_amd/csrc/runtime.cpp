@@ -7,6 +7,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "recon.hpp"
 #include "tomo.hpp"
 
 namespace moao {
@@ -64,7 +65,8 @@ static int init_device(int device) {
     // Default-priority streams only: mixing stream priorities can stall work indefinitely.
     MOAO_HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     g_ctx.device = device;
-    return MOAO_OK;
+    // the allocation-free reconstructor's scratch address on this device
+    return noalloc_scratch_setup();
 }
 
 int ensure_init() {

@@ -143,6 +143,11 @@ _SIGS = {
                                                     ctypes.c_void_p, ctypes.c_long,
                                                     ctypes.c_long, ctypes.c_void_p,
                                                     ctypes.c_void_p]),
+    "moao_hip_reconstructor_noalloc": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
+                                                      ctypes.c_void_p, ctypes.c_long,
+                                                      ctypes.c_void_p, ctypes.c_long,
+                                                      ctypes.c_long, ctypes.c_void_p,
+                                                      ctypes.c_void_p]),
     "moao_hip_reconstructor_mixed_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
                                                         ctypes.c_void_p, ctypes.c_long,
                                                         ctypes.c_void_p, ctypes.c_long,
