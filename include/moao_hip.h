@@ -127,7 +127,9 @@ int moao_hip_matcov_tile_dev(double* d_data, int nrows, int ncols, int xoffset, 
  * dpotrf('L', Cmm) then, per target g, Ctm_g = Ctm + g*nmeas*nmeasts is overwritten by
  * R_g = Ctm_g * Cmm^{-1} (dtrsm Right/Lower/Trans then Right/Lower/NoTrans), FP64.
  * Returns 0, the dpotrf info (> 0: leading minor of that order not positive definite; Cmm is
- * left partially factored and, for the host entry, Ctm untouched), or MOAO_ERR_*.            */
+ * left partially factored -- complete in the columns of the 128-column leaves before the
+ * failing pivot's leaf, and in that leaf's own rows of its 16-column blocks before the failing
+ * one -- and, for the host entry, Ctm untouched), or MOAO_ERR_*.            */
 int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, double* Ctm,
                            long ldCtm, long ngal);
 /* Device buffers; returns when the solve is done (its status is read back).  On info > 0 the

@@ -415,40 +415,43 @@ __global__ void __launch_bounds__(NT, TM == 128 ? 2 : 4) gemm_dl_kernel(GemmArgs
     }
 }
 
-// Split-BF16 kernel (FP32 storage).  Each operand element a is split as hi = bf16(a) and
+// Split-BF16 products.  Each FP32 operand element a is split as hi = bf16(a) and
 // lo = bf16(a - hi) (round to nearest even, v_cvt_pk_bf16_f32), and the product accumulates
 // lo*hi + hi*lo + hi*hi in FP32 on v_mfma_f32_16x16x32_bf16: about 16 significant bits per
 // operand (the dropped lo*lo term is 2^-16 relative) at three BF16 MFMAs per 16x16x32 step,
-// which together take 3/8 of the time of the eight FP32 MFMAs the same step costs.
-// The split runs once per element per tile, when the chunk is written to LDS: global loads of
-// chunk k+1 (FP32, registers) are issued before chunk k's MFMAs, then converted and stored as
-// hi / lo planes with k contiguous per row, so each MFMA operand (8 consecutive k of one row,
-// or of one column of op(B)) is a single 16-byte LDS read.  The MFMA pairs A's item i of lane
-// l with B's item i of the lane of the same l/16, so any common k order within a chunk gives
-// the same sum; the output layout is that of the FP32 16x16 MFMA (row 4*(l/16)+t, column l%16).
+// which together take 3/8 of the time of the eight FP32 MFMAs the same step costs.  The MFMA
+// pairs A's item i of lane l with B's item i of the lane of the same l/16, so any common k
+// order within a chunk gives the same sum; the output layout is that of the FP32 16x16 MFMA
+// (row 4*(l/16)+t, column l%16).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr int KC = 32;      // k per chunk (one 16x16x32 step)
-constexpr int KP = KC + 8;  // LDS row pitch in BF16 elements (80 bytes)
+constexpr int KC = 32;  // k per chunk (one 16x16x32 step)
 
-__device__ __forceinline__ void split8(const float* v, bf16x8& h, bf16x8& l) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const __bf16 x = (__bf16)v[i];
-        h[i] = x;
-        l[i] = (__bf16)(v[i] - (float)x);
-    }
+// Split-BF16 kernel on pre-split planes.  Each stage holds 32 k of the A tile (TM rows) and of
+// the B tile (128 rows of op(B)^T), hi and lo planes, in 16-row blocks of 1 KB laid out exactly
+// as the 16x16x32 MFMA takes its operands: lane l of a fragment read supplies row l % 16 and
+// k = 8 (l / 16) .. + 7, i.e. bytes [16 l, 16 l + 16) of the block, so every fragment is one
+// conflict-free 16-byte LDS read.  The blocks are filled by 16-byte buffer loads straight into
+// LDS (buffer_load_dwordx4 ... lds, one 1 KB block per instruction, no registers), kS2 stages
+// deep.  Products accumulate in FP32 in the MFMA; every kFlush stages the accumulators are
+// added into a second set of FP32 sums and restarted, so no accumulation chain runs longer
+// than 256 k.
+constexpr int kS2 = 3;      // stages of the planes kernel
+constexpr int kFlush = 8;   // stages between flushes of the MFMA accumulators
+
+__device__ __forceinline__ void dl_load16(__amdgpu_buffer_rsrc_t r, void* lds, int voff) {
+    __attribute__((address_space(3))) void* l = (__attribute__((address_space(3))) void*)lds;
+    asm("" : "+s"(l));
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, l, 16, voff, 0, 0, 0);
 }
 
-template <int TB, int TM>
-__global__ void __launch_bounds__(NT, 2) gemm_split_kernel(GemmArgs<float> g) {
-    constexpr int WMB = TM / 32;            // 16-row MFMA blocks per wave along M
-    constexpr int KA = KC * TM / NT;        // A values each thread loads per chunk (16 or 8)
-    constexpr int KB = KC * BN / NT;        // B values each thread loads per chunk (16)
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[TM * KP];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[TM * KP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * KP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bl[BN * KP];
+template <int TM>
+__global__ void __launch_bounds__(NT, TM == 128 ? 1 : 2) gemm_split2_kernel(GemmArgs<float> g) {
+    constexpr int WMB = TM / 32;          // 16-row MFMA blocks per wave along M
+    constexpr int ABLK = TM / 16;         // 16-row blocks of the A tile
+    constexpr int NBLK = ABLK + BN / 16;  // blocks per plane per stage
+    constexpr int IPW = 2 * NBLK / 4;     // 16-byte DMA instructions per wave per stage
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kS2][2][NBLK][1024];
 
     if (g.skip_if && *(volatile const int*)g.skip_if != 0) return;
     const GemmSeg<float> s = g.seg[blockIdx.z];
@@ -463,80 +466,65 @@ __global__ void __launch_bounds__(NT, 2) gemm_split_kernel(GemmArgs<float> g) {
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave & 1, wn = wave >> 1;
     const int K = g.K, N = g.N, M = s.M;
-    const int Kc = K > 0 ? K - 1 : 0;
 
-    f32x4 acc[WMB][4];
+    f32x4 acc[WMB][4], sum[WMB][4];
 #pragma unroll
     for (int i = 0; i < WMB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+        for (int j = 0; j < 4; ++j) acc[i][j] = sum[i][j] = f32x4{0, 0, 0, 0};
 
-    // A: thread -> row ra of the tile, k values [ka, ka + KA) of the chunk (lanes run down
-    // consecutive rows: every load instruction reads 64 consecutive floats of one column).
-    const int ra = tid % TM, ka = (tid / TM) * KA;
-    const bool ra_ok = m0 + ra < M;
-    const float* __restrict__ Ap = s.A + min(m0 + ra, M - 1);
-    // op(B) = B^T (B is N x K): as A.  op(B) = B (B is K x N): thread -> column rb, k values
-    // [kb, kb + KB) contiguous in memory.
-    const int rb = TB == 1 ? tid % BN : tid / 2;
-    const int kb = TB == 1 ? (tid / BN) * KB : (tid % 2) * KB;
-    const bool rb_ok = n0 + rb < N;
-    const float* __restrict__ Bp =
-        TB == 1 ? g.B + min(n0 + rb, N - 1) : g.B + (long)min(n0 + rb, N - 1) * g.ldb;
-    const long lda = s.lda, ldb = g.ldb;
-    float va[KA], vb[KB];
-    auto load = [&](int k0) {
+    // DMA instruction q of a stage: plane q / NBLK, block q % NBLK (A blocks, then B blocks).
+    // Lane l loads row (block row + l % 16), clamped into the operand, 16 bytes at k group l / 16.
+    const int lr = lane & 15, lkg = lane >> 4;
+    auto issue = [&](int kt) {
+        const int k0 = kt * KC;
 #pragma unroll
-        for (int r = 0; r < KA; ++r) {
-            const int k = k0 + ka + r;
-            const float x = Ap[(long)min(k, Kc) * lda];
-            va[r] = (ra_ok && k < K) ? x : 0.0f;
-        }
-#pragma unroll
-        for (int r = 0; r < KB; ++r) {
-            const int k = k0 + kb + r;
-            const float x = TB == 1 ? Bp[(long)min(k, Kc) * ldb] : Bp[min(k, Kc)];
-            vb[r] = (rb_ok && k < K) ? x : 0.0f;
-        }
-    };
-    auto store = [&]() {
-#pragma unroll
-        for (int q = 0; q < KA / 8; ++q) {
-            bf16x8 h, l;
-            split8(va + 8 * q, h, l);
-            *(bf16x8*)&Ah[ra * KP + ka + 8 * q] = h;
-            *(bf16x8*)&Al[ra * KP + ka + 8 * q] = l;
-        }
-#pragma unroll
-        for (int q = 0; q < KB / 8; ++q) {
-            bf16x8 h, l;
-            split8(vb + 8 * q, h, l);
-            *(bf16x8*)&Bh[rb * KP + kb + 8 * q] = h;
-            *(bf16x8*)&Bl[rb * KP + kb + 8 * q] = l;
+        for (int u = 0; u < IPW; ++u) {
+            const int q = wave + 4 * u;
+            const int pl = q / NBLK, blk = q % NBLK;
+            const bool isA = blk < ABLK;
+            const Planes& P = isA ? s.ap : g.bp;
+            const int rows = isA ? M : N;
+            const int rb = min(isA ? m0 + 16 * blk : n0 + 16 * (blk - ABLK), rows - 1);
+            const int rl = min(rb + lr, rows - 1) - rb;
+            const unsigned short* base = P.hi + (pl ? P.ps : 0) + (long)rb * P.ld + k0;
+            const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)base, 0, (int)(16 * P.ld * 2), 0x00020000);
+            dl_load16(r, &lds[kt % kS2][pl][blk][0], (rl * (int)P.ld + 8 * lkg) * 2);
         }
     };
 
     const int nk = (K + KC - 1) / KC;
-    const int ar = wm * 16 * WMB + (lane & 15);
-    const int bc = wn * 64 + (lane & 15);
-    const int kq = 8 * (lane >> 4);
-    if (nk > 0) {
-        load(0);
-        store();
-    }
-    __syncthreads();
+    for (int p = 0; p < kS2 - 1; ++p)
+        if (p < nk) issue(p);
     for (int kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) load((kt + 1) * KC);
+        if (kt + kS2 - 2 >= nk) wait_vm<0>();
+        else wait_vm<(kS2 - 2) * IPW>();
+        bar();
+        if (kt + kS2 - 1 < nk) issue(kt + kS2 - 1);
+        const int cur = kt % kS2;
         bf16x8 ah[WMB], al[WMB], bh[4], bl[4];
 #pragma unroll
         for (int i = 0; i < WMB; ++i) {
-            ah[i] = *(const bf16x8*)&Ah[(ar + 16 * i) * KP + kq];
-            al[i] = *(const bf16x8*)&Al[(ar + 16 * i) * KP + kq];
+            ah[i] = *(const bf16x8*)&lds[cur][0][wm * WMB + i][16 * lane];
+            al[i] = *(const bf16x8*)&lds[cur][1][wm * WMB + i][16 * lane];
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            bh[j] = *(const bf16x8*)&Bh[(bc + 16 * j) * KP + kq];
-            bl[j] = *(const bf16x8*)&Bl[(bc + 16 * j) * KP + kq];
+            bh[j] = *(const bf16x8*)&lds[cur][0][ABLK + wn * 4 + j][16 * lane];
+            bl[j] = *(const bf16x8*)&lds[cur][1][ABLK + wn * 4 + j][16 * lane];
+        }
+        if (kt * KC + KC > K) {
+            // last chunk: k past K reads whatever follows in the plane row; zero it on both sides
+            const int kv = K - kt * KC - 8 * lkg;
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                if (e >= kv) {
+#pragma unroll
+                    for (int i = 0; i < WMB; ++i) ah[i][e] = al[i][e] = (__bf16)0.0f;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) bh[j][e] = bl[j][e] = (__bf16)0.0f;
+                }
         }
 #pragma unroll
         for (int i = 0; i < WMB; ++i)
@@ -546,10 +534,14 @@ __global__ void __launch_bounds__(NT, 2) gemm_split_kernel(GemmArgs<float> g) {
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
             }
-        if (kt + 1 < nk) {
-            __syncthreads();
-            store();
-            __syncthreads();
+        if ((kt % kFlush) == kFlush - 1) {
+#pragma unroll
+            for (int i = 0; i < WMB; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    sum[i][j] += acc[i][j];
+                    acc[i][j] = f32x4{0, 0, 0, 0};
+                }
         }
     }
 
@@ -558,6 +550,7 @@ __global__ void __launch_bounds__(NT, 2) gemm_split_kernel(GemmArgs<float> g) {
     for (int i = 0; i < WMB; ++i) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
+            const f32x4 tot = sum[i][j] + acc[i][j];
             const int col = n0 + wn * 64 + j * 16 + (lane & 15);
             if (col >= N) continue;
 #pragma unroll
@@ -566,10 +559,50 @@ __global__ void __launch_bounds__(NT, 2) gemm_split_kernel(GemmArgs<float> g) {
                 if (row >= M) continue;
                 if (tri && row + g.tri_off < col) continue;
                 float* c = s.C + row + (long)col * s.ldc;
-                float v = alpha * acc[i][j][t];
+                float v = alpha * tot[t];
                 if (beta != 0.0f) v += beta * *c;
                 *c = v;
             }
+        }
+    }
+}
+
+// Planes of a column-major FP32 region: T = 1 -> plane row r, k = c; T = 0 -> plane row c, k = r.
+// A 64 x 64 tile passes through LDS so both the FP32 reads and the plane writes are coalesced.
+template <int T>
+__global__ void split_planes_kernel(const float* __restrict__ A, long lda, int rows, int cols,
+                                    unsigned short* __restrict__ hi, long ld, long ps) {
+    __shared__ float tile[64][65];
+    const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+    if (T == 0) {
+        // same orientation: element (r, c) -> hi[c * ld + r]; read and write down columns
+        for (int c = ty; c < 64; c += 4) {
+            const int gr = r0 + tx, gc = c0 + c;
+            if (gr < rows && gc < cols) {
+                const float v = A[gr + (long)gc * lda];
+                const __bf16 h = (__bf16)v;
+                const __bf16 l = (__bf16)(v - (float)h);
+                hi[(long)gc * ld + gr] = __builtin_bit_cast(unsigned short, h);
+                hi[ps + (long)gc * ld + gr] = __builtin_bit_cast(unsigned short, l);
+            }
+        }
+        return;
+    }
+    for (int c = ty; c < 64; c += 4) {
+        const int gr = r0 + tx, gc = c0 + c;
+        tile[c][tx] = (gr < rows && gc < cols) ? A[gr + (long)gc * lda] : 0.0f;
+    }
+    __syncthreads();
+    // transposed: element (r, c) -> hi[r * ld + c]; lanes run along c
+    for (int r = ty; r < 64; r += 4) {
+        const int gr = r0 + r, gc = c0 + tx;
+        if (gr < rows && gc < cols) {
+            const float v = tile[tx][r];
+            const __bf16 h = (__bf16)v;
+            const __bf16 l = (__bf16)(v - (float)h);
+            hi[(long)gr * ld + gc] = __builtin_bit_cast(unsigned short, h);
+            hi[ps + (long)gr * ld + gc] = __builtin_bit_cast(unsigned short, l);
         }
     }
 }
@@ -615,25 +648,23 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
     // 128-row tiles when they fill the device (>= 2 per CU), else 64-row tiles
     const bool small = (long)grid.x * grid.y * grid.z < 2L * 256;
     if constexpr (std::is_same<T, float>::value) {
-        if (g.split) {
-            if (g.transb == 2) {
-                set_error("gemm: the split-BF16 product takes op(B) = B or B^T only");
+        if (g.split == 2) {
+            if (g.transb != 1 && g.transb != 0) {
+                set_error("gemm: the planes product takes op(B) = B or B^T only");
                 return MOAO_ERR_ARG;
             }
-            if (small) grid.x = (maxM + 63) / 64;
-            if (g.transb == 1) {
-                if (small)
-                    hipLaunchKernelGGL((gemm_split_kernel<1, 64>), grid, dim3(NT), 0, stream, g);
-                else
-                    hipLaunchKernelGGL((gemm_split_kernel<1, 128>), grid, dim3(NT), 0, stream, g);
+            if (small) {
+                grid.x = (maxM + 63) / 64;
+                hipLaunchKernelGGL((gemm_split2_kernel<64>), grid, dim3(NT), 0, stream, g);
             } else {
-                if (small)
-                    hipLaunchKernelGGL((gemm_split_kernel<0, 64>), grid, dim3(NT), 0, stream, g);
-                else
-                    hipLaunchKernelGGL((gemm_split_kernel<0, 128>), grid, dim3(NT), 0, stream, g);
+                hipLaunchKernelGGL((gemm_split2_kernel<128>), grid, dim3(NT), 0, stream, g);
             }
             MOAO_HIP_TRY(hipGetLastError());
             return MOAO_OK;
+        }
+        if (g.split) {
+            set_error("gemm: unknown split mode");
+            return MOAO_ERR_ARG;
         }
     }
     // the K x N slab loads address 128 columns with 32-bit byte offsets
@@ -668,6 +699,21 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
 
 int gemm_f64(const GemmArgs<double>& a, hipStream_t stream) { return launch(a, stream); }
 int gemm_f32(const GemmArgs<float>& a, hipStream_t stream) { return launch(a, stream); }
+
+int split_planes(const float* A, long lda, int rows, int cols, unsigned short* hi, long ld,
+                 long ps, int transposed, hipStream_t stream) {
+    if (rows <= 0 || cols <= 0) return MOAO_OK;
+    ProfScope pscope(kProfConvert, 8.0 * rows * cols, stream);
+    const dim3 grid((rows + 63) / 64, (cols + 63) / 64);
+    if (transposed)
+        hipLaunchKernelGGL(split_planes_kernel<1>, grid, dim3(256), 0, stream, A, lda, rows, cols,
+                           hi, ld, ps);
+    else
+        hipLaunchKernelGGL(split_planes_kernel<0>, grid, dim3(256), 0, stream, A, lda, rows, cols,
+                           hi, ld, ps);
+    MOAO_HIP_TRY(hipGetLastError());
+    return MOAO_OK;
+}
 
 size_t gemm_symdiag_elems(long K) { return (size_t)((K + BN - 1) / BN) * BN * BN; }
 
@@ -740,8 +786,32 @@ int moao_hip_sgemm_bf16x3_dev(int transb, int M, int N, int K, float alpha, cons
         return MOAO_ERR_ARG;
     }
     auto a = moao::gemm_args<float>(M, N, K, alpha, A, lda, B, ldb, transb, beta, C, ldc);
-    a.split = 1;
-    return moao::gemm_f32(a, moao::pick_stream(stream));
+    hipStream_t s = moao::pick_stream(stream);
+    if (M == 0 || N == 0) return MOAO_OK;
+    // both operands split into planes (stream-ordered scratch), then the planes kernel
+    const long ld = ((long)std::max(K, 1) + 63) / 64 * 64;
+    const long psa = (long)M * ld, psb = (long)N * ld;
+    unsigned short* pl = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&pl), sizeof(unsigned short) * 2 * (psa + psb),
+                       s) != hipSuccess) {
+        (void)hipGetLastError();
+        moao::set_error("sgemm_bf16x3: plane allocation failed");
+        return MOAO_ERR_NOMEM;
+    }
+    unsigned short* pa = pl;
+    unsigned short* pb = pl + 2 * psa;
+    int rc = moao::split_planes(A, lda, M, K, pa, ld, psa, 1, s);
+    if (rc == MOAO_OK)
+        rc = transb == 1 ? moao::split_planes(B, ldb, N, K, pb, ld, psb, 1, s)
+                         : moao::split_planes(B, ldb, K, N, pb, ld, psb, 0, s);
+    if (rc == MOAO_OK) {
+        a.split = 2;
+        a.seg[0].ap = moao::Planes{pa, ld, psa};
+        a.bp = moao::Planes{pb, ld, psb};
+        rc = moao::gemm_f32(a, s);
+    }
+    (void)hipFreeAsync(pl, s);
+    return rc;
 }
 
 }  // extern "C"

@@ -22,12 +22,22 @@ namespace moao {
 
 constexpr int kMaxSeg = 4;
 
+// Split-BF16 planes of an FP32 matrix region: the hi term of element (row, k) at
+// hi[row * ld + k], the lo term at hi[ps + row * ld + k] (k contiguous within a row; ld a
+// multiple of 8 and at least the row's last chunk of 32 k).
+struct Planes {
+    const unsigned short* hi;
+    long ld;
+    long ps;
+};
+
 template <typename T>
 struct GemmSeg {
     const T* A;
     T* C;
     long lda, ldc;
     int M;
+    Planes ap;  // split = 2: A's planes (row = row of this segment)
 };
 
 template <typename T>
@@ -44,10 +54,11 @@ struct GemmArgs {
     // transb = 2 only, optional: full symmetric copies of B's 128 x 128 diagonal blocks
     // (gemm_symdiag); with it the launch runs on the direct-to-LDS kernel.
     const T* symdiag;
-    // FP32 only: 1 = run the product on BF16 MFMA with each operand split into two BF16 terms
-    // (hi = bf16(a), lo = bf16(a - hi); hi*hi + hi*lo + lo*hi accumulated in FP32), op(B) = B or
-    // B^T; 0 = FP32 MFMA.
+    // FP32 only: 2 = run the product on BF16 MFMA from split-BF16 planes of the operands
+    // (hi = bf16(a), lo = bf16(a - hi); hi*hi + hi*lo + lo*hi accumulated in FP32; A from
+    // seg[].ap, op(B) element (k, n) from bp row n); 0 = FP32 MFMA.
     int split;
+    Planes bp;  // split = 2: B's planes (row = column n of op(B))
     // Optional device status word: when it is non-zero at launch time the whole launch is a
     // no-op.  The factorization points it at its dpotrf info, so the updates and solves queued
     // after a failing pivot do nothing and no host synchronisation is needed to stop them.
@@ -60,6 +71,12 @@ size_t gemm_symdiag_elems(long K);
 // matrix whose lower triangle B holds; what a transb = 2 launch reads where its K slices cross
 // the diagonal.
 int gemm_symdiag(const double* B, long ldb, int K, double* out, hipStream_t stream);
+
+// Planes of an FP32 column-major region (rows x cols at A, leading dimension lda):
+// transposed = 1 writes element (r, c) as plane row r, k = c (hi[r * ld + c]); transposed = 0 as
+// plane row c, k = r (hi[c * ld + r]).
+int split_planes(const float* A, long lda, int rows, int cols, unsigned short* hi, long ld,
+                 long ps, int transposed, hipStream_t stream);
 
 // Launches on `stream`; returns a MOAO_* status (argument checks only, launch is async).
 int gemm_f64(const GemmArgs<double>& a, hipStream_t stream);
@@ -86,6 +103,7 @@ inline GemmArgs<T> gemm_args(int M, int N, int K, T alpha, const T* A, long lda,
     g.beta = beta;
     g.symdiag = nullptr;
     g.split = 0;
+    g.bp = Planes{nullptr, 0, 0};
     g.skip_if = nullptr;
     return g;
 }

@@ -1,5 +1,9 @@
 // Mixed-precision reconstructor: R = Ctm * Cmm^{-1} to a stated tolerance with the O(n^3)
-// factorization and the O(m n^2) solves in FP32 and the residuals in FP64.
+// factorization and the O(m n^2) solves in low precision and the residuals in FP64.  Two modes:
+// MOAO_MIXED_FP32 runs the updates on FP32 MFMA; MOAO_MIXED_BF16 on BF16 MFMA with each FP32
+// operand split into two BF16 terms (split-BF16 planes of every finished column of L and of the
+// right-hand sides are written as the recursion goes, and of L by columns before the second
+// solve), FP32 accumulation; the leaves are FP64 in both.
 //
 // Cmm carries an exact low-rank term: the reference adds lgs_cst to every entry of each LGS
 // xx and yy block (common/kernels/matcov_kernels.cpp:320-321), i.e. Cmm = C0 + c U U^T with U
@@ -7,12 +11,12 @@
 // ill-conditioned (its eigenvalues c*Nsubap sit 4-6 decades above the noise floor; FP32
 // Cholesky of Cmm loses definiteness by n ~ 1e4) while C0 stays well conditioned, so:
 //   1. C0 (FP32) = Cmm - c U U^T, converted in one pass;
-//   2. FP32 tall factorization of C0 with [Ctm; U^T] riding along, then the second solve:
-//      Y = Ctm C0^{-1}, W = U^T C0^{-1}   (FP32 MFMA updates, FP64 leaves);
+//   2. tall factorization of C0 (FP32 storage) with [Ctm; U^T] riding along, then the second
+//      solve: Y = Ctm C0^{-1}, W = U^T C0^{-1}   (low-precision updates, FP64 leaves);
 //   3. Woodbury in FP64: S = I/c + W U (p x p, host), R = Y - (Y U) S^{-1} W;
 //   4. refinement: E = Ctm - R Cmm (FP64 GEMM on the full Cmm, lower triangle read);
-//      stop when ||E||_F <= tol ||Ctm||_F, else D = E C0^{-1} (FP32 solves) corrected by
-//      Woodbury, R += D.
+//      stop when ||E||_F <= tol ||Ctm||_F, else D = E C0^{-1} (low-precision solves) corrected
+//      by Woodbury, R += D.
 // Without a low-rank description (p = 0) the same loop runs on Cmm itself.
 // With the environment variable MOAO_DEBUG set, the norms of the intermediate terms and the
 // residual of every refinement step are printed to stderr (synchronising after each).
@@ -173,8 +177,13 @@ struct MixedWs {
 
 }  // namespace
 
-// Byte offsets of the workspace pieces (MixedWs order) and the total size.
-static size_t mixed_layout(long n, long m, int p, std::vector<size_t>* offs) {
+// Leading dimension of the split-BF16 planes of an order-n solve.
+static long plane_ld(long n) { return (n + 63) / 64 * 64; }
+
+// Byte offsets of the workspace pieces (MixedWs order, then the symmetric diagonal blocks and,
+// for the BF16 mode, the planes of L in both orientations and of the right-hand sides) and the
+// total size.
+static size_t mixed_layout(long n, long m, int p, int mode, std::vector<size_t>* offs) {
     const long xr = m + p;
     const long q = std::max(p, 1);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -198,11 +207,17 @@ static size_t mixed_layout(long n, long m, int p, std::vector<size_t>* offs) {
     take(sizeof(int) * n);
     take(sizeof(int) * 4);
     take(sizeof(double) * gemm_symdiag_elems(n));  // Cmm's symmetric diagonal blocks
+    if (mode == MOAO_MIXED_BF16) {
+        const long ld = plane_ld(n);
+        take(sizeof(unsigned short) * 2 * n * ld);   // L, plane row = row
+        take(sizeof(unsigned short) * 2 * n * ld);   // L, plane row = column
+        take(sizeof(unsigned short) * 2 * xr * ld);  // right-hand sides
+    }
     return off;
 }
 
-size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank) {
-    return mixed_layout(n, m, lowrank_ranges(lowrank, n).p(), nullptr);
+size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank, int mode) {
+    return mixed_layout(n, m, lowrank_ranges(lowrank, n).p(), mode, nullptr);
 }
 
 int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, double* Ctm, long ldCtm,
@@ -221,7 +236,7 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
     const int p = rg.p();
     const long xr = m + p;  // rows of the FP32 right-hand-side block
     std::vector<size_t> offs;
-    const size_t total = mixed_layout(n, m, p, &offs);
+    const size_t total = mixed_layout(n, m, p, mode, &offs);
     if (!ws || ws_bytes < total) {
         set_error("reconstructor_mixed: workspace too small");
         return MOAO_ERR_ARG;
@@ -234,6 +249,14 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
               (int*)(b + offs[12]), (int*)(b + offs[13])};
     double* const symd = (double*)(b + offs[14]);
     const int N = (int)n, M = (int)m;
+    SplitWs sw{};
+    const SplitWs* swp = nullptr;
+    if (split) {
+        const long ld = plane_ld(n);
+        sw = SplitWs{(unsigned short*)(b + offs[15]), (unsigned short*)(b + offs[16]),
+                     (unsigned short*)(b + offs[17]), ld, n * ld, n * ld, xr * ld};
+        swp = &sw;
+    }
 
     auto norm = [&](const double* a, long lda, int rows, int cols, double* out) -> int {
         hipLaunchKernelGGL(sumsq_partial, dim3(kNormBlocks), dim3(256), 0, s, a, lda, rows, cols,
@@ -290,7 +313,7 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
 
     // --- 2. FP32 factorization with [Ctm; U^T] riding along, second solve
     std::vector<RowGroup<float>> all{{w.X32, xr, (int)xr}};
-    if (int rc = potrf_tall<float>(n, w.C0, n, all, w.invL, w.info, s, split)) return rc;
+    if (int rc = potrf_tall<float>(n, w.C0, n, all, w.invL, w.info, s, swp)) return rc;
     int hinfo = 0;
     MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
     MOAO_HIP_TRY(hipStreamSynchronize(s));
@@ -298,7 +321,10 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
         set_error("reconstructor_mixed: FP32 factorization of C0 failed");
         return hinfo;
     }
-    if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, all, s, nullptr, split))
+    // the second solve reads L by columns: its planes with plane row = column of L
+    if (split)
+        if (int rc = split_planes(w.C0, n, N, N, sw.lq, sw.ld, sw.lqs, 0, s)) return rc;
+    if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, all, s, nullptr, swp))
         return rc;
 
     // --- 3. Woodbury terms
@@ -373,11 +399,11 @@ int reconstructor_mixed_dev(long n, long m, const double* Cmm, long ldCmm, doubl
         // D = E C0^{-1} (FP32 solves), Woodbury-corrected in FP64, R += D
         hipLaunchKernelGGL(cvt_f64_f32, grid2(M, N), dim3(256), 0, s, w.E, m, w.X32, xr, M, N);
         std::vector<RowGroup<float>> rhs{{w.X32, xr, M}};
-        if (int rc = trsm_right_lower_trans<float>(n, w.C0, n, w.invL, rhs, s, nullptr, split)) {
+        if (int rc = trsm_right_lower_trans<float>(n, w.C0, n, w.invL, rhs, s, nullptr, swp)) {
             status = rc;
             break;
         }
-        if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, rhs, s, nullptr, split)) {
+        if (int rc = trsm_right_lower_notrans<float>(n, w.C0, n, w.invL, rhs, s, nullptr, swp)) {
             status = rc;
             break;
         }
@@ -414,7 +440,7 @@ int moao_hip_reconstructor_mixed_dev(long nmeas, long nmeasts, const double* dCm
     }
     // workspace of this call only, from the stream-ordered pool
     hipStream_t s = moao::pick_stream(stream);
-    const size_t bytes = moao::mixed_workspace_bytes(nmeas, nmeasts, lowrank);
+    const size_t bytes = moao::mixed_workspace_bytes(nmeas, nmeasts, lowrank, mode);
     void* ws = nullptr;
     if (hipMallocAsync(&ws, bytes, s) != hipSuccess) {
         (void)hipGetLastError();

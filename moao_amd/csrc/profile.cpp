@@ -35,6 +35,9 @@ bool profiling() { return g_on; }
 
 int prof_start(int cls, double work, hipStream_t s) {
     if (!g_on) return -1;
+    // launches recorded into a graph are not timed here (no event records in a capture)
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return -1;
     std::lock_guard<std::mutex> g(g_mu);
     hipEvent_t a = take(), b = take();
     if (!a || !b) return -1;

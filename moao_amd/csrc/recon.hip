@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -48,7 +49,7 @@ struct Tall {
     int* info;
     const int* skip;  // device status word: launches are no-ops once it is non-zero
     hipStream_t s;
-    int split;        // FP32 storage: update products on the split-BF16 kernel
+    const SplitWs* sw;  // FP32 storage: update products on the split-BF16 planes kernel
     T* a(long r, long c) const { return A + r + c * lda; }
     T* inv(long j0) const { return invL + (j0 / kLeaf) * (long)kLeaf * kLeaf; }
 };
@@ -80,6 +81,28 @@ int over_groups(const std::vector<RowGroup<T>>& groups, GemmArgs<T> proto, bool 
     return MOAO_OK;
 }
 
+// Planes of the columns [j0, j0 + nn) just finalised, for the split-BF16 updates: rows
+// [r0, n) of the SPD block (lead) and every row of the (single) row group.
+template <typename T>
+int split_cols_done(const Tall<T>& p, long r0, long j0, long nn, bool lead) {
+    if constexpr (std::is_same<T, float>::value) {
+        const SplitWs& w = *p.sw;
+        if (lead && p.n > r0)
+            if (int rc = split_planes(p.a(r0, j0), p.lda, (int)(p.n - r0), (int)nn,
+                                      w.lp + r0 * w.ld + j0, w.ld, w.lps, 1, p.s))
+                return rc;
+        for (const auto& g : *p.groups)
+            if (g.rows > 0)
+                if (int rc = split_planes(g.p + j0 * g.ld, g.ld, g.rows, (int)nn, w.xp + j0, w.ld,
+                                          w.xps, 1, p.s))
+                    return rc;
+        return MOAO_OK;
+    } else {
+        set_error("split-BF16 planes need FP32 storage");
+        return MOAO_ERR_ARG;
+    }
+}
+
 // Factor columns [j0, j0+nn) of the tall matrix (all earlier columns already applied).
 template <typename T>
 int potrf_rec(const Tall<T>& p, long j0, long nn) {
@@ -99,14 +122,19 @@ int potrf_rec(const Tall<T>& p, long j0, long nn) {
         proto.transb = 1;
         proto.alpha = T(1);
         proto.beta = T(0);
-        return over_groups<T>(
-            G, proto, below > 0,
-            [&] { return GemmSeg<T>{p.a(j0 + nn, j0), p.a(j0 + nn, j0), p.lda, p.lda, (int)below}; },
-            [&](const RowGroup<T>& g) {
-                T* c = g.p + j0 * g.ld;
-                return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
-            },
-            p.s);
+        if (int rc = over_groups<T>(
+                G, proto, below > 0,
+                [&] {
+                    return GemmSeg<T>{p.a(j0 + nn, j0), p.a(j0 + nn, j0), p.lda, p.lda,
+                                      (int)below};
+                },
+                [&](const RowGroup<T>& g) {
+                    T* c = g.p + j0 * g.ld;
+                    return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
+                },
+                p.s))
+            return rc;
+        return p.sw ? split_cols_done(p, j0, j0, nn, true) : MOAO_OK;
     }
     const long n1 = split(nn);
     if (int rc = potrf_rec(p, j0, n1)) return rc;
@@ -123,12 +151,21 @@ int potrf_rec(const Tall<T>& p, long j0, long nn) {
     proto.tri_off = 0;
     proto.alpha = T(-1);
     proto.beta = T(1);
-    proto.split = p.split;
+    if (p.sw) {
+        proto.split = 2;
+        proto.bp = p.sw->l_rows(r0, j0);
+    }
     int rc = over_groups<T>(
         G, proto, true,
-        [&] { return GemmSeg<T>{p.a(r0, j0), p.a(r0, r0), p.lda, p.lda, (int)(p.n - r0)}; },
+        [&] {
+            GemmSeg<T> sg{p.a(r0, j0), p.a(r0, r0), p.lda, p.lda, (int)(p.n - r0)};
+            if (p.sw) sg.ap = p.sw->l_rows(r0, j0);
+            return sg;
+        },
         [&](const RowGroup<T>& g) {
-            return GemmSeg<T>{g.p + j0 * g.ld, g.p + r0 * g.ld, g.ld, g.ld, g.rows};
+            GemmSeg<T> sg{g.p + j0 * g.ld, g.p + r0 * g.ld, g.ld, g.ld, g.rows};
+            if (p.sw) sg.ap = p.sw->x_rows(j0);
+            return sg;
         },
         p.s);
     if (rc) return rc;
@@ -150,12 +187,14 @@ int trsmN_rec(const Tall<T>& p, long j0, long nn) {
         proto.transb = 0;
         proto.alpha = T(1);
         proto.beta = T(0);
-        return over_groups<T>(G, proto, false, none,
-                              [&](const RowGroup<T>& g) {
-                                  T* c = g.p + j0 * g.ld;
-                                  return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
-                              },
-                              p.s);
+        if (int rc = over_groups<T>(G, proto, false, none,
+                                    [&](const RowGroup<T>& g) {
+                                        T* c = g.p + j0 * g.ld;
+                                        return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
+                                    },
+                                    p.s))
+            return rc;
+        return p.sw ? split_cols_done(p, 0, j0, nn, false) : MOAO_OK;
     }
     const long n1 = split(nn);
     if (int rc = trsmN_rec(p, j0 + n1, nn - n1)) return rc;
@@ -169,11 +208,16 @@ int trsmN_rec(const Tall<T>& p, long j0, long nn) {
     proto.transb = 0;
     proto.alpha = T(-1);
     proto.beta = T(1);
-    proto.split = p.split;
+    if (p.sw) {
+        proto.split = 2;
+        proto.bp = p.sw->l_cols(j0, j0 + n1);
+    }
     int rc = over_groups<T>(G, proto, false, none,
                             [&](const RowGroup<T>& g) {
-                                return GemmSeg<T>{g.p + (j0 + n1) * g.ld, g.p + j0 * g.ld, g.ld,
-                                                  g.ld, g.rows};
+                                GemmSeg<T> sg{g.p + (j0 + n1) * g.ld, g.p + j0 * g.ld, g.ld, g.ld,
+                                              g.rows};
+                                if (p.sw) sg.ap = p.sw->x_rows(j0 + n1);
+                                return sg;
                             },
                             p.s);
     if (rc) return rc;
@@ -195,12 +239,14 @@ int trsmT_rec(const Tall<T>& p, long j0, long nn) {
         proto.transb = 1;
         proto.alpha = T(1);
         proto.beta = T(0);
-        return over_groups<T>(G, proto, false, none,
-                              [&](const RowGroup<T>& g) {
-                                  T* c = g.p + j0 * g.ld;
-                                  return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
-                              },
-                              p.s);
+        if (int rc = over_groups<T>(G, proto, false, none,
+                                    [&](const RowGroup<T>& g) {
+                                        T* c = g.p + j0 * g.ld;
+                                        return GemmSeg<T>{c, c, g.ld, g.ld, g.rows};
+                                    },
+                                    p.s))
+            return rc;
+        return p.sw ? split_cols_done(p, 0, j0, nn, false) : MOAO_OK;
     }
     const long n1 = split(nn);
     if (int rc = trsmT_rec(p, j0, n1)) return rc;
@@ -214,11 +260,16 @@ int trsmT_rec(const Tall<T>& p, long j0, long nn) {
     proto.transb = 1;
     proto.alpha = T(-1);
     proto.beta = T(1);
-    proto.split = p.split;
+    if (p.sw) {
+        proto.split = 2;
+        proto.bp = p.sw->l_rows(j0 + n1, j0);
+    }
     int rc = over_groups<T>(G, proto, false, none,
                             [&](const RowGroup<T>& g) {
-                                return GemmSeg<T>{g.p + j0 * g.ld, g.p + (j0 + n1) * g.ld, g.ld,
-                                                  g.ld, g.rows};
+                                GemmSeg<T> sg{g.p + j0 * g.ld, g.p + (j0 + n1) * g.ld, g.ld, g.ld,
+                                              g.rows};
+                                if (p.sw) sg.ap = p.sw->x_rows(j0);
+                                return sg;
                             },
                             p.s);
     if (rc) return rc;
@@ -231,8 +282,12 @@ __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 template <typename T>
 int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
-               int* info, hipStream_t s, int split) {
-    Tall<T> p{n, A, lda, &groups, invL, info, info, s, split};
+               int* info, hipStream_t s, const SplitWs* sw) {
+    if (sw && groups.size() > 1) {
+        set_error("split-BF16 planes take one row group");
+        return MOAO_ERR_ARG;
+    }
+    Tall<T> p{n, A, lda, &groups, invL, info, info, s, sw};
     // a kernel rather than a memset node, so the sequence can be captured into a graph
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, info, 0);
     MOAO_HIP_TRY(hipGetLastError());
@@ -242,35 +297,35 @@ int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T
 template <typename T>
 int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
                              const std::vector<RowGroup<T>>& groups, hipStream_t s,
-                             const int* skip, int split) {
-    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s, split};
+                             const int* skip, const SplitWs* sw) {
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s, sw};
     return trsmN_rec(p, 0, n);
 }
 
 template <typename T>
 int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
                            const std::vector<RowGroup<T>>& groups, hipStream_t s,
-                           const int* skip, int split) {
-    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s, split};
+                           const int* skip, const SplitWs* sw) {
+    Tall<T> p{n, const_cast<T*>(A), lda, &groups, const_cast<T*>(invL), nullptr, skip, s, sw};
     return trsmT_rec(p, 0, n);
 }
 
 template int potrf_tall<double>(long, double*, long, const std::vector<RowGroup<double>>&,
-                                double*, int*, hipStream_t, int);
+                                double*, int*, hipStream_t, const SplitWs*);
 template int potrf_tall<float>(long, float*, long, const std::vector<RowGroup<float>>&, float*,
-                               int*, hipStream_t, int);
+                               int*, hipStream_t, const SplitWs*);
 template int trsm_right_lower_notrans<double>(long, const double*, long, const double*,
                                               const std::vector<RowGroup<double>>&, hipStream_t,
-                                              const int*, int);
+                                              const int*, const SplitWs*);
 template int trsm_right_lower_notrans<float>(long, const float*, long, const float*,
                                              const std::vector<RowGroup<float>>&, hipStream_t,
-                                             const int*, int);
+                                             const int*, const SplitWs*);
 template int trsm_right_lower_trans<double>(long, const double*, long, const double*,
                                             const std::vector<RowGroup<double>>&, hipStream_t,
-                                            const int*, int);
+                                            const int*, const SplitWs*);
 template int trsm_right_lower_trans<float>(long, const float*, long, const float*,
                                            const std::vector<RowGroup<float>>&, hipStream_t,
-                                           const int*, int);
+                                           const int*, const SplitWs*);
 
 int reconstructor_f64_dev(long n, long m, double* dCmm, long ldCmm, double* dCtm, long ldCtm,
                           long ngal, double* dinvL, int* dinfo, hipStream_t stream) {
@@ -482,8 +537,9 @@ int moao_hip_reconstructor(long nmeas, long nmeasts, double* Cmm, long ldCmm, do
     int hinfo = 0;
     MOAO_HIP_TRY(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
     MOAO_HIP_TRY(hipStreamSynchronize(s));
-    // Cmm gets the factor (on a failing pivot: the columns of the leaves before it, and the
-    // 16-column blocks before the failing one within its leaf); R only on success.
+    // Cmm gets the factor (on a failing pivot: every row of the columns of the leaves before
+    // it, and the failing leaf's own rows of its 16-column blocks before the failing one; the
+    // panel rows below a leaf are solved only once the leaf is whole); R only on success.
     MOAO_HIP_TRY(hipMemcpyAsync(Cmm, dA, a_bytes, hipMemcpyDeviceToHost, s));
     if (hinfo == 0 && b_elems)
         MOAO_HIP_TRY(hipMemcpyAsync(Ctm, dB, b_elems * sizeof(double), hipMemcpyDeviceToHost, s));

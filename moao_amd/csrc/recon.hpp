@@ -5,9 +5,25 @@
 
 #include <vector>
 
+#include "gemm.hpp"
 #include "leaf.hpp"
 
 namespace moao {
+
+// Split-BF16 planes of the mixed BF16 solve (FP32 storage): lp holds L (n x n) with plane row
+// r = row of L and k = column, lq holds L with plane row = column of L and k = row, xp the
+// (single) row group with plane row = group row and k = column; all with leading dimension ld
+// (a multiple of 64, >= n), hi terms at p[...], lo terms at p[ps + ...].
+struct SplitWs {
+    unsigned short* lp;
+    unsigned short* lq;
+    unsigned short* xp;
+    long ld, lps, lqs, xps;
+    // operand views: rows from r0, k from c0
+    Planes l_rows(long r0, long c0) const { return Planes{lp + r0 * ld + c0, ld, lps}; }
+    Planes l_cols(long c0, long r0) const { return Planes{lq + c0 * ld + r0, ld, lqs}; }
+    Planes x_rows(long c0) const { return Planes{xp + c0, ld, xps}; }
+};
 
 // A block of rows that rides along the factorization as extra rows of the tall matrix
 // [Cmm; group_0; group_1; ...]: after potrf_tall each group holds X = B * L^{-T}.
@@ -21,22 +37,23 @@ struct RowGroup {
 // Recursive factorization of the n x n SPD block (lower, in place) with the row groups
 // solved by L^T on the way (first right-side solve fused).  invL receives the inverse of every
 // kLeaf leaf (ceil(n/kLeaf)*kLeaf*kLeaf elements); *info the dpotrf status (device int).
-// split (FP32 storage only): 1 runs the update products (not the leaves' panel products) on
-// the split-BF16 GEMM kernel.
+// sw (FP32 storage only): run the update products (not the leaves' panel products) on the
+// split-BF16 planes kernel, writing the planes of every finished leaf column into sw->lp and
+// sw->xp (one row group at most).
 template <typename T>
 int potrf_tall(long n, T* A, long lda, const std::vector<RowGroup<T>>& groups, T* invL,
-               int* info, hipStream_t s, int split = 0);
+               int* info, hipStream_t s, const SplitWs* sw = nullptr);
 // Y * L = X for every group (second right-side solve), with L / invL from potrf_tall.
 // skip: optional device status word; every launch is a no-op once it is non-zero.
 template <typename T>
 int trsm_right_lower_notrans(long n, const T* A, long lda, const T* invL,
                              const std::vector<RowGroup<T>>& groups, hipStream_t s,
-                             const int* skip = nullptr, int split = 0);
+                             const int* skip = nullptr, const SplitWs* sw = nullptr);
 // X * L^T = B for every group (first right-side solve, standalone).
 template <typename T>
 int trsm_right_lower_trans(long n, const T* A, long lda, const T* invL,
                            const std::vector<RowGroup<T>>& groups, hipStream_t s,
-                           const int* skip = nullptr, int split = 0);
+                           const int* skip = nullptr, const SplitWs* sw = nullptr);
 
 // FP64 path, stream-ordered (no host synchronisation).  dCmm (n x n, lower) is overwritten by
 // its Cholesky factor L; each of the ngal target blocks Ctm_g = dCtm + g*n*m (m x n, leading

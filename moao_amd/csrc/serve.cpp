@@ -304,7 +304,7 @@ int moao_serve_submit(moao_server* s, const moao_serve_request* q, long* ticket)
     r->R = q->R;
     r->ldR = q->ldR;
     r->solve_bytes = r->mode == MOAO_FP64 ? moao::recon_workspace_bytes(r->n)
-                                          : moao::mixed_workspace_bytes(r->n, r->m, &r->view);
+                                          : moao::mixed_workspace_bytes(r->n, r->m, &r->view, r->mode);
     r->footprint = moao::al256(sizeof(double) * r->n * r->n) +
                    moao::al256(sizeof(double) * r->m * r->n) + moao::al256(r->solve_bytes);
     r->res.n = r->n;

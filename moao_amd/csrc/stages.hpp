@@ -16,7 +16,7 @@ int matcov_tile_launch(double* d_data, int nrows, int ncols, int xoffset, int yo
                        const moao_tomo_view* t, int type_mat, hipStream_t stream, DevBuf* tables);
 
 // Bytes of scratch the mixed-precision reconstructor needs for this problem.
-size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank);
+size_t mixed_workspace_bytes(long n, long m, const moao_tomo_view* lowrank, int mode);
 
 // Mixed-precision reconstructor (FP32 factorization of C0 = Cmm - lgs_cst U U^T, Woodbury,
 // FP64 refinement to tol).  Cmm is read only; Ctm is overwritten by R.  ws: device scratch of
