@@ -158,6 +158,19 @@ int moao_hip_reconstructor_noalloc(long nmeas, long nmeasts, double* dCmm, long 
                                    double* dCtm, long ldCtm, long ngal, int* d_info,
                                    void* stream);
 
+/* A captured FP64 solve: the launch sequence of moao_hip_reconstructor_async on these buffers
+ * recorded once as a HIP graph (no allocation, memset or host synchronisation inside; the leaf
+ * inverses live in workspace the graph object owns) and replayed by solve_graph_launch on any
+ * stream.  Each launch solves whatever Cmm / Ctm hold at that point (refill them in between, as
+ * the reference's refine loop regenerates them); the status goes to *d_info.  No counterpart
+ * in the reference: for callers that solve one shape repeatedly, where launch work is a large
+ * part of a small solve.  A graph must not run on two streams at once (shared workspace). */
+typedef struct moao_solve_graph moao_solve_graph;
+int moao_hip_solve_graph_create(long nmeas, long nmeasts, double* dCmm, long ldCmm, double* dCtm,
+                                long ldCtm, long ngal, int* d_info, moao_solve_graph** out);
+int moao_hip_solve_graph_launch(moao_solve_graph* g, void* stream);
+void moao_hip_solve_graph_destroy(moao_solve_graph* g);
+
 /* The same solve on FP32 storage: MOAO_CHAMELEON::reconstructor<float>, the precision the
  * reference's Chameleon driver runs in production (chameleon/moao_chameleon.cpp:41,
  * chameleon/compute/reconstructor.cpp:31-65: spotrf + 2 strsm).  FP32 MFMA updates, the
@@ -251,6 +264,19 @@ int moao_hip_intersample_process_d(moao_isample_struct* s, const double* Cvv);
 int moao_hip_intersample_process_f(moao_isample_struct* s, const float* Cvv);
 void moao_hip_intersample_free(moao_isample_struct* s);
 
+/* ---------------------------------------------------------------- accuracy check ----------
+ * compareMatrices (common/compute/check_utils.cpp:20-57), the reference check harness's metric:
+ * max_err = max |1 - B/A| over the visited entries (|B| where A == 0), frob = sqrt of the sum of
+ * (B - A)^2 (off-diagonal terms doubled for uplo 'U' / 'L').  A is column-major nrows x ncols;
+ * B's entry (i, j) is B[i + j*nrows] when trans == 'T', else B[i*ncols + j].  uplo 'U' visits
+ * j >= i, 'L' j < i, anything else every entry.  The _dev variant reads device buffers (one
+ * reduction launch) and returns when the result is on the host.                            */
+int moao_compare_matrices(const double* A, const double* B, long nrows, long ncols, char uplo,
+                          char trans, double* max_err, double* frob);
+int moao_hip_compare_matrices_dev(const double* dA, const double* dB, long nrows, long ncols,
+                                  char uplo, char trans, double* max_err, double* frob,
+                                  void* stream);
+
 /* ---------------------------------------------------------------- FITS images -------------
  * The reference's FITS layout (common/io/fits.cpp:66-148): a primary image, NAXIS1 the fast
  * axis, memory order on disk.  read: any of BITPIX -64/-32/8/16/32/64 as doubles; out == NULL
@@ -326,6 +352,8 @@ typedef struct moao_serve_result {
 typedef struct moao_serve_counters {
     long queued, running, completed, peak_running;
     size_t bytes_in_use, peak_bytes, budget;
+    long graph_captures;   /* FP64 solves recorded as graphs (a slot's solve seen twice) */
+    long graph_replays;    /* FP64 solves run by replaying a slot's graph */
 } moao_serve_counters;
 
 int moao_serve_create(const moao_serve_config* cfg, moao_server** out);

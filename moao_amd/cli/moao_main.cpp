@@ -11,7 +11,19 @@
 // Cmm, then per target Ctm, Ctt (type 4), Cee/Cvv and the PSF (psf_oloop*_iloop*_gal*.fits in
 // --out).  Matrices stay in device memory between stages; --out also receives R.fits,
 // Cee.fits and Cvv.fits of the last step (FITS images of doubles, memory order, as writeFits).
-// --mode mixed solves each target with the mixed-precision reconstructor instead.
+// --mode mixed (FP32 updates) or bf16 (split-BF16 updates) solves each target with the
+// mixed-precision reconstructor instead.  With --psf 1 the targets' PSFs of one observation
+// step are computed in one batched call, and the long-exposure PSF of each target (the mean of
+// its short-exposure PSFs over every refine and observation step, which the reference leaves
+// as a TODO at chameleon/moao_chameleon.cpp:324-345) is written as
+// psf_nbrefine{maxrefine}_maxobs{maxobs}_gal{g}.fits.
+//
+// Check mode (lapack/check_moao_lapack.cpp, also selected by running the binary as
+// check_moao): --sys_check DIR [--compare_all 0|1] [--write_all 0|1] [--accuracy 1e-7]
+// [--out_path DIR] runs one pass Cmm -> Ctm -> R -> Cmm, Ctm, Ctt -> Cee/Cvv -> PSF on
+// prof-1-atmos-night0.txt, compares each stage with DIR/{Cmm,Ctm,R,Ctt,Cee,Cvv,psf}.fits by
+// compareMatrices (all of them with --compare_all 1, the PSF only otherwise), prints the
+// reference's table and exits 1 when a maximum error exceeds the accuracy.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -19,6 +31,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -144,7 +157,9 @@ struct Args {
     std::map<std::string, std::string> kv{{"maxrefine", "1"}, {"maxobs", "1"},
                                           {"snapPerNight", "1"}, {"nact", "0"},
                                           {"verbose", "0"}, {"mode", "fp64"},
-                                          {"tol", "1e-11"}, {"psf", "0"}, {"out", ""}};
+                                          {"tol", "1e-11"}, {"psf", "0"}, {"out", ""},
+                                          {"compare_all", "0"}, {"write_all", "0"},
+                                          {"accuracy", "1.e-7"}, {"out_path", ""}};
     std::string s(const char* k) { return kv[k]; }
     long i(const char* k) { return std::atol(kv[k].c_str()); }
 };
@@ -161,6 +176,145 @@ std::string prof(const std::string& atm, long idx) {
     return atm + "/prof" + std::to_string(idx) + "-atmos-night0.txt";
 }
 
+int mode_of(const std::string& m) {
+    if (m == "fp64") return MOAO_FP64;
+    if (m == "mixed") return MOAO_MIXED_FP32;
+    if (m == "bf16") return MOAO_MIXED_BF16;
+    std::fprintf(stderr, "moao: unknown --mode %s (fp64, mixed or bf16)\n", m.c_str());
+    std::exit(2);
+}
+
+// One check pass (lapack/check_moao_lapack.cpp:28-273) over the C-ABI.
+int run_check(Args& a) {
+    const std::string sys = a.s("sys_path"), atm = a.s("atm_path"), chk = a.s("sys_check");
+    const std::string outp = a.s("out_path");
+    const bool compare_all = a.i("compare_all") != 0, write_all = a.i("write_all") != 0;
+    const double accuracy = std::atof(a.s("accuracy").c_str());
+    Timer t_Cmm, t_Ctm, t_Ctt, t_R, t_CeeCvv, t_psf, t_glob;
+    t_glob.start();
+    check(moao_hip_init(0), "init");
+    moao_tomo* tomo = nullptr;
+    if (moao_tomo_create((sys + "/sys-params.txt").c_str(), (atm + "/prof-1-atmos-night0.txt").c_str(),
+                         &tomo))
+        die("tomo");
+    const long n = moao_tomo_nmeas(tomo), m = moao_tomo_nmeasts(tomo);
+    std::printf("using %ld meas and %ld ts meas\n", n, m);
+    long dims[6];
+    moao_tomo_dims(tomo, dims);
+    std::vector<double> Dx, idxd, a2f, otf;
+    long nact = 0, dxm = 0, nidx = 0, one = 0, N = 0, ny = 0, ox = 0, oy = 0;
+    const bool isample = fits_read(sys + "/Dx.fits", &nact, &dxm, &Dx) &&
+                         fits_read(sys + "/idx.fits", &nidx, &one, &idxd) &&
+                         fits_read(sys + "/abs2fi.fits", &N, &ny, &a2f) &&
+                         fits_read(sys + "/otftel.fits", &ox, &oy, &otf);
+    if (isample) std::printf("Using %ld actuators\n", nact);
+    // matrices start zeroed (the reference callocs them: Cmm's strict upper part stays 0)
+    auto zeroed = [&](size_t cnt) {
+        auto d = std::make_unique<Dev>(std::max<size_t>(cnt, 1));
+        std::vector<double> z(std::max<size_t>(cnt, 1), 0.0);
+        check(moao_hip_memcpy_h2d(d->p, z.data(), z.size() * 8), "zero");
+        return d;
+    };
+    auto dCmm = zeroed((size_t)n * n), dR = zeroed((size_t)m * n), dCtm = zeroed((size_t)m * n),
+         dCtt = zeroed((size_t)m * m), dCee = zeroed((size_t)m * m),
+         dCvv = zeroed((size_t)nact * nact), dDx = zeroed((size_t)nact * m),
+         dTmp = zeroed((size_t)nact * m), dRef = zeroed((size_t)std::max(n * n, std::max(nact * nact, m * n)));
+    std::map<std::string, std::pair<double, double>> err;  // stage -> (max, frobenius)
+    auto compare = [&](const char* name, const double* d, long nr, long nc, bool always) {
+        if (!outp.empty() && write_all) {
+            std::vector<double> h((size_t)nr * nc);
+            check(moao_hip_memcpy_d2h(h.data(), d, h.size() * 8), "download");
+            fits_write(outp + "/" + name + ".fits", nr, nc, h.data());
+        }
+        if (!compare_all && !always) return;
+        long rx = 0, ry = 0;
+        std::vector<double> ref;
+        if (!fits_read(chk + "/" + name + ".fits", &rx, &ry, &ref) || rx * ry != nr * nc)
+            die(std::string("reference file ") + chk + "/" + name + ".fits missing or misshaped");
+        check(moao_hip_memcpy_h2d(dRef->p, ref.data(), ref.size() * 8), "upload");
+        double mx = 0, fr = 0;
+        check(moao_hip_compare_matrices_dev(d, dRef->p, nr, nc, 'N', 'T', &mx, &fr, nullptr),
+              "compareMatrices");
+        err[name] = {mx, fr};
+    };
+    moao_tomo_view view;
+    moao_tomo_get_view(tomo, &view);
+    auto stage = [&](Timer& t, const char* label, auto&& body) {
+        std::printf("%-8s", label);
+        std::fflush(stdout);
+        t.start();
+        body();
+        check(moao_hip_synchronize(), "sync");
+        t.stop();
+        std::printf("done\n");
+    };
+    stage(t_Cmm, "Cmm", [&] { check(moao_hip_matcov_tile_dev(dCmm->p, n, n, 0, 0, n, &view, 1, nullptr), "Cmm"); });
+    compare("Cmm", dCmm->p, n, n, false);
+    moao_tomo_set_target(tomo, 0);
+    moao_tomo_get_view(tomo, &view);
+    stage(t_Ctm, "Ctm", [&] { check(moao_hip_matcov_tile_dev(dR->p, m, n, 0, 0, m, &view, 3, nullptr), "Ctm"); });
+    compare("Ctm", dR->p, m, n, false);
+    stage(t_R, "R", [&] { check(moao_hip_reconstructor_dev(n, m, dCmm->p, n, dR->p, m, 1, nullptr), "R"); });
+    compare("R", dR->p, m, n, false);
+    // Cmm and Ctm again (destroyed by the reconstructor), then Ctt
+    stage(t_Cmm, "Cmm", [&] { check(moao_hip_matcov_tile_dev(dCmm->p, n, n, 0, 0, n, &view, 1, nullptr), "Cmm"); });
+    stage(t_Ctm, "Ctm", [&] { check(moao_hip_matcov_tile_dev(dCtm->p, m, n, 0, 0, m, &view, 3, nullptr), "Ctm"); });
+    stage(t_Ctt, "Ctt", [&] { check(moao_hip_matcov_tile_dev(dCtt->p, m, m, 0, 0, m, &view, 4, nullptr), "Ctt"); });
+    compare("Ctt", dCtt->p, m, m, false);
+    if (isample) {
+        check(moao_hip_memcpy_h2d(dDx->p, Dx.data(), Dx.size() * 8), "Dx");
+        stage(t_CeeCvv, "Cee,Cvv", [&] {
+            check(moao_hip_cee_cvv_dev(n, m, nact, dCmm->p, n, dCtt->p, m, dCtm->p, m, dR->p, m,
+                                       dDx->p, nact, dCee->p, m, dCvv->p, nact, dTmp->p, nact,
+                                       nullptr), "Cee_Cvv");
+        });
+        compare("Cee", dCee->p, m, m, false);
+        compare("Cvv", dCvv->p, nact, nact, false);
+        std::vector<double> ns(dims[0]);
+        moao_tomo_field(tomo, "Nssp", ns.data(), dims[0]);
+        std::vector<long> idx(idxd.begin(), idxd.end());
+        moao_isample* is = nullptr;
+        // FP64 transforms: the reference's FFTW arithmetic
+        check(moao_hip_intersample_create(nidx, (long)ns[dims[0] - 1] + 1, N, idx.data(),
+                                          a2f.data(), otf.data(), 1, &is), "intersample");
+        N = moao_hip_intersample_n(is);
+        auto dPsf = zeroed((size_t)N * N);
+        stage(t_psf, "psf", [&] {
+            check(moao_hip_intersample_process_dev(is, dCvv->p, 1, dPsf->p, nullptr), "psf");
+        });
+        if (!outp.empty()) {
+            std::vector<double> h((size_t)N * N);
+            check(moao_hip_memcpy_d2h(h.data(), dPsf->p, h.size() * 8), "psf");
+            fits_write(outp + "/psf.fits", N, N, h.data());
+        }
+        compare("psf", dPsf->p, N, N, true);
+        moao_hip_intersample_destroy(is);
+    }
+    t_glob.stop();
+    moao_tomo_destroy(tomo);
+    if (!compare_all) std::printf("WARNING: error only for psf!\n");
+    auto e = [&](const char* k, int w) {
+        auto it = err.find(k);
+        return it == err.end() ? 0.0 : (w ? it->second.first : it->second.second);
+    };
+    std::printf("==============================\n");
+    std::printf("operation: Cmm           Ctm           Ctt           R             Cee           "
+                "Cvv           psf          total\n");
+    std::printf("time (s) : %e  %e  %e  %e  %e                %e %e\n", t_Cmm.total,
+                t_Ctm.total, t_Ctt.total, t_R.total, t_CeeCvv.total, t_psf.total, t_glob.total);
+    std::printf("F norm   : %e  %e  %e  %e  %e  %e  %e    \n", e("Cmm", 0), e("Ctm", 0),
+                e("Ctt", 0), e("R", 0), e("Cee", 0), e("Cvv", 0), e("psf", 0));
+    std::printf("max err  : %e  %e  %e  %e  %e  %e  %e    \n", e("Cmm", 1), e("Ctm", 1),
+                e("Ctt", 1), e("R", 1), e("Cee", 1), e("Cvv", 1), e("psf", 1));
+    std::printf("==============================\n");
+    for (const auto& kv : err)
+        if (kv.second.first > accuracy) {
+            std::fprintf(stderr, "test failed: bad accuracy\n\t expected %g\n", accuracy);
+            return 1;
+        }
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -172,16 +326,28 @@ int main(int argc, char** argv) {
         }
         a.kv[argv[k] + 2] = argv[k + 1];
     }
+    const std::string prog = argv[0];
+    const bool as_check = prog.size() >= 10 && prog.compare(prog.size() - 10, 10, "check_moao") == 0;
+    if (a.kv.count("sys_check") || as_check) {
+        if (!a.kv.count("sys_path") || !a.kv.count("atm_path") || !a.kv.count("sys_check")) {
+            std::fprintf(stderr, "usage: check_moao --sys_path DIR --atm_path DIR --sys_check DIR "
+                                 "[--compare_all 0|1] [--write_all 0|1] [--accuracy 1e-7] "
+                                 "[--out_path DIR]\n");
+            return 2;
+        }
+        return run_check(a);
+    }
     if (!a.kv.count("sys_path") || !a.kv.count("atm_path")) {
         std::fprintf(stderr,
                      "usage: moao --sys_path DIR --atm_path DIR [--maxrefine 1] [--maxobs 1] "
-                     "[--snapPerNight 1] [--nact N] [--verbose 0] [--mode fp64|mixed] "
+                     "[--snapPerNight 1] [--nact N] [--verbose 0] [--mode fp64|mixed|bf16] "
                      "[--tol 1e-11] [--psf 0|1] [--out DIR]\n");
         return 2;
     }
     const std::string sys = a.s("sys_path"), atm = a.s("atm_path"), out = a.s("out");
     const long maxrefine = a.i("maxrefine"), maxobs = a.i("maxobs"), spn = a.i("snapPerNight");
-    const bool verbose = a.i("verbose") != 0, mixed = a.s("mode") == "mixed";
+    const int mode = mode_of(a.s("mode"));
+    const bool verbose = a.i("verbose") != 0, mixed = mode != MOAO_FP64;
     const double tol = std::atof(a.s("tol").c_str());
     Timer t_matcov, t_R, t_CeeCvv, t_psf, t_pre, t_glob;
     t_glob.start();
@@ -226,9 +392,12 @@ int main(int argc, char** argv) {
     t_pre.stop();
 
     Dev dCmm((size_t)n * n), dR((size_t)m * n * nTarget), dCtm((size_t)m * n),
-        dCtt((size_t)m * m), dCee((size_t)m * m), dCvv((size_t)std::max(nact, 1L) * std::max(nact, 1L)),
+        dCtt((size_t)m * m), dCee((size_t)m * m),
+        dCvv((size_t)std::max(nact, 1L) * std::max(nact, 1L) * nTarget),
         dDx((size_t)std::max(nact, 1L) * m), dTmp((size_t)std::max(nact, 1L) * m),
-        dPsf((size_t)std::max(N, 1L) * std::max(N, 1L));
+        dPsf((size_t)std::max(N, 1L) * std::max(N, 1L) * nTarget);
+    // long-exposure PSF of each target: the sum of its short-exposure PSFs
+    std::vector<double> cumul(isample ? (size_t)N * N * nTarget : 0, 0.0);
     if (nact) check(moao_hip_memcpy_h2d(dDx.p, Dx.data(), Dx.size() * 8), "Dx upload");
     moao_tomo_view view;
 
@@ -255,7 +424,7 @@ int main(int argc, char** argv) {
                 double resid;
                 int it;
                 check(moao_hip_reconstructor_mixed_dev(n, m, dCmm.p, n, dR.p + g * n * m, m,
-                                                       &view, MOAO_MIXED_FP32, tol, 8, &resid,
+                                                       &view, mode, tol, 8, &resid,
                                                        &it, nullptr),
                       "reconstructor (mixed)");
                 if (verbose) std::printf("target %ld: %d refinement steps, residual %.3e\n", g,
@@ -282,29 +451,41 @@ int main(int argc, char** argv) {
                 check(moao_hip_synchronize(), "sync");
                 t_matcov.stop();
                 t_CeeCvv.start();
+                const long na = std::max(nact, 1L);
                 check(moao_hip_cee_cvv_dev(n, m, nact, dCmm.p, n, dCtt.p, m, dCtm.p, m,
-                                           dR.p + g * n * m, m, dDx.p, std::max(nact, 1L),
-                                           dCee.p, m, dCvv.p, std::max(nact, 1L), dTmp.p,
-                                           std::max(nact, 1L), nullptr),
+                                           dR.p + g * n * m, m, dDx.p, na, dCee.p, m,
+                                           dCvv.p + g * na * na, na, dTmp.p, na, nullptr),
                       "Cee_Cvv");
                 check(moao_hip_synchronize(), "sync");
                 t_CeeCvv.stop();
-                if (isample) {
-                    t_psf.start();
-                    check(moao_hip_intersample_process_dev(isample, dCvv.p, 1, dPsf.p, nullptr),
-                          "intersample");
-                    std::vector<double> psf((size_t)N * N);
-                    check(moao_hip_memcpy_d2h(psf.data(), dPsf.p, psf.size() * 8), "psf");
-                    t_psf.stop();
-                    if (!out.empty())
+            }
+            if (isample) {
+                // every target's PSF of this step in one batched call
+                t_psf.start();
+                check(moao_hip_intersample_process_dev(isample, dCvv.p, (int)nTarget, dPsf.p,
+                                                       nullptr),
+                      "intersample");
+                std::vector<double> psf((size_t)N * N * nTarget);
+                check(moao_hip_memcpy_d2h(psf.data(), dPsf.p, psf.size() * 8), "psf");
+                t_psf.stop();
+                for (size_t e = 0; e < psf.size(); ++e) cumul[e] += psf[e];
+                if (!out.empty())
+                    for (long g = 0; g < nTarget; ++g)
                         fits_write(out + "/psf_oloop" + std::to_string(r) + "_iloop" +
                                        std::to_string(o) + "_gal" + std::to_string(g) + ".fits",
-                                   N, N, psf.data());
-                }
+                                   N, N, psf.data() + (size_t)g * N * N);
             }
         }
     }
     t_glob.stop();
+    if (isample && !out.empty() && maxrefine * maxobs > 0) {
+        const double w = 1.0 / (double)(maxrefine * maxobs);
+        for (auto& v : cumul) v *= w;
+        for (long g = 0; g < nTarget; ++g)
+            fits_write(out + "/psf_nbrefine" + std::to_string(maxrefine) + "_maxobs" +
+                           std::to_string(maxobs) + "_gal" + std::to_string(g) + ".fits",
+                       N, N, cumul.data() + (size_t)g * N * N);
+    }
     if (!out.empty()) {
         std::vector<double> h((size_t)m * n * nTarget);
         check(moao_hip_memcpy_d2h(h.data(), dR.p, h.size() * 8), "R");
@@ -328,7 +509,7 @@ int main(int argc, char** argv) {
                 "========================================\ntimes (s)\n"
                 "preprocess: %.3f\nmatcov    : %.3f\nR         : %.3f\nCee_Cvv   : %.3f\n"
                 "psf       : %.3f\ntotal     : %.3f\n",
-                moao_version(), n, m, nact, dims[4], maxrefine, maxobs, mixed ? "mixed" : "fp64",
+                moao_version(), n, m, nact, dims[4], maxrefine, maxobs, a.s("mode").c_str(),
                 t_pre.total, t_matcov.total, t_R.total, t_CeeCvv.total, t_psf.total,
                 t_glob.total);
     return 0;

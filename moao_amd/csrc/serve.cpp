@@ -11,7 +11,12 @@
 //     large requests are not starved);
 //   * a slot allocates the request's device block stream-ordered (hipMallocAsync from the
 //     device pool, which keeps freed memory cached), generates Cmm and Ctm with matcov, solves
-//     FP64 or mixed precision, copies R out if asked, and frees the block on its stream.
+//     FP64 or mixed precision, copies R out if asked, and frees the block on its stream;
+//   * an FP64 solve of order up to kGraphMaxN that repeats the slot's previous solve (same
+//     shape, and the pool handed back the same block -- the common case for a stream of one
+//     system's refine steps) runs as a captured graph: one hipGraphLaunch instead of its launch
+//     sequence.  The slot records the graph the second time it sees that solve and replays it
+//     after; a shape seen once runs eagerly, so one-off requests pay no capture.
 // Work of different slots overlaps on the device; the kernels, their order and their inputs for
 // one request do not depend on what else runs, so a result is bit-identical alone or in a mix.
 #include <hip/hip_runtime.h>
@@ -27,6 +32,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "graph.hpp"
 #include "recon.hpp"
 #include "stages.hpp"
 #include "tomo.hpp"
@@ -80,9 +86,17 @@ struct Request {
     std::string error;
 };
 
+constexpr long kGraphMaxN = 16384;  // FP64 solves up to this order run as captured graphs
+
 struct Slot {
     hipStream_t stream = nullptr;
-    DevBuf tables;  // matcov upload tables (small, kept)
+    hipStream_t capture = nullptr;       // idle stream the slot records its graphs on
+    DevBuf tables;                       // matcov upload tables (small, kept)
+    std::unique_ptr<SolveGraph> graph;   // the graph of this slot's last captured solve
+    struct {
+        long n = 0, m = 0;
+        const void *Cmm = nullptr, *Ctm = nullptr, *invL = nullptr, *info = nullptr;
+    } last;                              // shape and buffers of the slot's previous FP64 solve
 };
 
 }  // namespace
@@ -97,6 +111,7 @@ struct moao_server {
     std::vector<std::thread> workers;
     size_t budget = 0, in_use = 0, peak_bytes = 0;
     long running = 0, peak_running = 0, next_ticket = 1, completed = 0;
+    long graph_captures = 0, graph_replays = 0;
     int max_bypass = 8;
     bool stop = false;
     moao::Clock::time_point t0 = moao::Clock::now();
@@ -152,8 +167,39 @@ int moao_server::run_body(moao::Request& r, moao::Slot& sl) {
         if (r.mode == MOAO_FP64) {
             double* invL = reinterpret_cast<double*>(scratch);
             int* info = reinterpret_cast<int*>(scratch + recon_workspace_bytes(n) - 256);
-            rc = reconstructor_f64_dev(n, m, Cmm, n, Ctm, m, 1, invL, info, s);
-            if (rc) break;
+            const bool again = sl.last.n == n && sl.last.m == m && sl.last.Cmm == Cmm &&
+                               sl.last.Ctm == Ctm && sl.last.invL == invL && sl.last.info == info;
+            sl.last.n = n;
+            sl.last.m = m;
+            sl.last.Cmm = Cmm;
+            sl.last.Ctm = Ctm;
+            sl.last.invL = invL;
+            sl.last.info = info;
+            const bool replay = sl.graph && sl.graph->matches(n, m, 1, Cmm, Ctm, invL, info);
+            if (n <= kGraphMaxN && (replay || again)) {
+                if (replay) {
+                    std::lock_guard<std::mutex> g(mu);
+                    ++graph_replays;
+                } else {
+                    sl.graph.reset(new SolveGraph);
+                    rc = capture_solve(n, m, Cmm, n, Ctm, m, 1, invL, info, sl.capture,
+                                       sl.graph.get());
+                    if (rc) {
+                        sl.graph.reset();
+                        break;
+                    }
+                    std::lock_guard<std::mutex> g(mu);
+                    ++graph_captures;
+                }
+                if (hipGraphLaunch(sl.graph->exec, s) != hipSuccess) {
+                    set_error("serve: graph launch failed");
+                    rc = MOAO_ERR_HIP;
+                    break;
+                }
+            } else {
+                rc = reconstructor_f64_dev(n, m, Cmm, n, Ctm, m, 1, invL, info, s);
+                if (rc) break;
+            }
             if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s) !=
                 hipSuccess) {
                 set_error("serve: status copy failed");
@@ -253,9 +299,12 @@ int moao_serve_create(const moao_serve_config* cfg, moao_server** out) {
     if (cfg && cfg->max_bypass > 0) s->max_bypass = cfg->max_bypass;
     s->slots.resize(nslots);
     for (auto& sl : s->slots) {
-        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) {
-            for (auto& o : s->slots)
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&sl.capture, hipStreamNonBlocking) != hipSuccess) {
+            for (auto& o : s->slots) {
                 if (o.stream) (void)hipStreamDestroy(o.stream);
+                if (o.capture) (void)hipStreamDestroy(o.capture);
+            }
             delete s;
             moao::set_error("serve: stream creation failed");
             return MOAO_ERR_HIP;
@@ -352,6 +401,8 @@ int moao_serve_stats(moao_server* s, moao_serve_counters* st) {
     st->peak_bytes = s->peak_bytes;
     st->peak_running = s->peak_running;
     st->budget = s->budget;
+    st->graph_captures = s->graph_captures;
+    st->graph_replays = s->graph_replays;
     return MOAO_OK;
 }
 
@@ -367,7 +418,9 @@ void moao_serve_destroy(moao_server* s) {
     for (auto& t : s->workers) t.join();
     for (auto& sl : s->slots) {
         sl.tables.release();
+        sl.graph.reset();
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.capture) (void)hipStreamDestroy(sl.capture);
     }
     delete s;
 }

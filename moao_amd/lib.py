@@ -82,7 +82,8 @@ class ServeCounters(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_long), ("running", ctypes.c_long),
                 ("completed", ctypes.c_long), ("peak_running", ctypes.c_long),
                 ("bytes_in_use", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t),
-                ("budget", ctypes.c_size_t)]
+                ("budget", ctypes.c_size_t), ("graph_captures", ctypes.c_long),
+                ("graph_replays", ctypes.c_long)]
 
 
 _lib = None
@@ -148,6 +149,12 @@ _SIGS = {
                                                       ctypes.c_void_p, ctypes.c_long,
                                                       ctypes.c_long, ctypes.c_void_p,
                                                       ctypes.c_void_p]),
+    "moao_hip_solve_graph_create": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_void_p,
+                                                   ctypes.c_long, ctypes.c_void_p, ctypes.c_long,
+                                                   ctypes.c_long, ctypes.c_void_p,
+                                                   ctypes.POINTER(ctypes.c_void_p)]),
+    "moao_hip_solve_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "moao_hip_solve_graph_destroy": (None, [ctypes.c_void_p]),
     "moao_hip_reconstructor_mixed_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
                                                         ctypes.c_void_p, ctypes.c_long,
                                                         ctypes.c_void_p, ctypes.c_long,
@@ -192,6 +199,13 @@ _SIGS = {
     "moao_hip_intersample_process_f": (ctypes.c_int, [ctypes.POINTER(IsampleStruct),
                                                       ctypes.POINTER(ctypes.c_float)]),
     "moao_hip_intersample_free": (None, [ctypes.POINTER(IsampleStruct)]),
+    "moao_compare_matrices": (ctypes.c_int, [_c_double_p, _c_double_p, ctypes.c_long,
+                                             ctypes.c_long, ctypes.c_char, ctypes.c_char,
+                                             _c_double_p, _c_double_p]),
+    "moao_hip_compare_matrices_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.c_long, ctypes.c_long, ctypes.c_char,
+                                                     ctypes.c_char, _c_double_p, _c_double_p,
+                                                     ctypes.c_void_p]),
     "moao_fits_read": (ctypes.c_int, [ctypes.c_char_p, _c_long_p, _c_long_p, _c_double_p,
                                       ctypes.c_long]),
     "moao_fits_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_long, ctypes.c_long,

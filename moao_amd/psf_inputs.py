@@ -9,6 +9,9 @@ and the telescope OTF, as the reference's input pipeline defines them.
                  the Gaussian influence function sampled at ud = 3 D / N
   make_otf_tel   cython/pipeline/intersample.pyx:174-200: autocorrelation of the annular pupil
                  normalised so the diffraction-limited PSF peak (Strehl) is 1
+  interaction_matrix / command_matrix
+                 cython/pipeline/commande.py:81-127: the truth sensor's interaction matrix from
+                 the influence function and Dx as its truncated-SVD pseudo-inverse
 Arrays are returned flat in the reference's (FITS) memory order.
 """
 from __future__ import annotations
@@ -75,3 +78,43 @@ def make_otf_tel(N: int, diam: float, obs: float) -> np.ndarray:
     ac = np.fft.ifft2(b * b).real
     otf = ac / surface_pix ** 2
     return np.ascontiguousarray(otf.T).flatten()
+
+
+def actuator_positions(dm: DM):
+    """(csX, csY) of the valid actuators in pupil-radius units (commande.py:61-70)."""
+    lin = np.linspace(-1, 1, dm.ndiam)
+    return lin[dm.csI % dm.ndiam], lin[dm.csI // dm.ndiam]
+
+
+def interaction_matrix(X: np.ndarray, Y: np.ndarray, nssp: int, diam: float, dm: DM) -> np.ndarray:
+    """Interaction matrix of the truth sensor (commande.py:81-106): slopes (x slopes of every
+    valid subaperture, then y slopes; arcsec) per volt of each actuator, from the difference of
+    the Gaussian influence function (funcInflu, commande.py:160-174) across each subaperture.
+    X, Y: the sensor's valid subaperture centres in metres, in the system's order."""
+    sx, sy = np.asarray(X) / (diam / 2.0), np.asarray(Y) / (diam / 2.0)
+    cx, cy = actuator_positions(dm)
+    dx = sx[None, :] - cx[:, None]          # actuator x subaperture
+    dy = sy[None, :] - cy[:, None]
+    ss = (2.0 / nssp) / 2.0
+
+    def fi(x, y):
+        return 1.0e-6 * np.exp(-(x * x + y * y) / (2 * dm.x0 * dm.x0))
+
+    nsub = sx.size
+    mia = np.zeros((2 * nsub, dm.nact))
+    mia[:nsub, :] = (fi(dx + ss, dy) - fi(dx - ss, dy)).T
+    mia[nsub:, :] = (fi(dy + ss, dx) - fi(dy - ss, dx)).T
+    return mia / (diam / nssp) * 206265.0
+
+
+def command_matrix(mia: np.ndarray, thresh: float = 0.0, condi: float = 0.0) -> np.ndarray:
+    """Dx = the truncated-SVD pseudo-inverse of the interaction matrix (computeCommandMat,
+    commande.py:108-127): singular values below thresh (max/sqrt(condi) when condi is set,
+    1e-4 when neither is) are filtered.  Shape nact x 2 nsub, column-major, FP64."""
+    U, w, Vt = np.linalg.svd(mia, full_matrices=False)
+    if condi != 0.0:
+        thresh = np.max(w) / np.sqrt(condi)
+    if thresh == 0.0:
+        thresh = 1e-4
+    winv = np.where(w < thresh, 0.0, 1.0 / np.where(w < thresh, 1.0, w))
+    return np.asfortranarray((Vt.T * winv) @ U.T)

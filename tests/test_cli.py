@@ -68,3 +68,108 @@ def test_refine_and_observation_step(tmp_path):
     ref = oi.process(Cvvf, idx, a2f, otf, N, dm.ndiam)
     assert pd == (N, N)
     assert np.max(np.abs(psf.reshape(N, N) - ref)) <= 2e-5 * np.max(np.abs(ref))
+
+
+def _psf_system(d, nprof=2):
+    from moao_amd import fits, psf_inputs, systems
+    from moao_amd.tomo import Tomo
+    from oracle import intersample as oi
+    s, atms = systems.make_system(d, "small", nprof=nprof)
+    t = Tomo(s, atms[0])
+    n, m = t.nmeas(), t.nmeasts()
+    dm = psf_inputs.make_dm(10)
+    nact = dm.nact
+    Dx = systems.dx_matrix(nact, m, n)
+    fits.write_image(os.path.join(d, "Dx.fits"), Dx, nact, m)
+    idx = psf_inputs.make_idx(dm)
+    N = oi.default_n(dm.ndiam)
+    a2f = psf_inputs.make_abs2fi(N, 8.0, dm)
+    otf = psf_inputs.make_otf_tel(N, 8.0, 0.28)
+    fits.write_image(os.path.join(d, "idx.fits"), idx, idx.size, 1)
+    fits.write_image(os.path.join(d, "abs2fi.fits"), a2f, N, N)
+    fits.write_image(os.path.join(d, "otftel.fits"), otf, N, N)
+    return s, atms, t, n, m, dm, Dx, idx, a2f, otf, N
+
+
+def test_check_usage():
+    r = subprocess.run([os.path.join(ROOT, "moao_amd", "_lib", "check_moao")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "--sys_check" in r.stderr
+
+
+@pytest.mark.gpu
+def test_check_mode_table_and_gate(tmp_path):
+    """check_moao (lapack/check_moao_lapack.cpp): every stage of one pass compared with
+    reference files -- here written by the oracle chain (matcov restatement, scipy LAPACK path,
+    Cee/Cvv closed form, numpy PSF) -- by compareMatrices.  The table's maximum errors and
+    Frobenius norms equal the oracle's own compare_matrices on the same matrices (the check
+    harness measures what it says), the gate passes at an accuracy the stages meet and fails,
+    with exit status 1, once one reference file is wrong."""
+    import re
+    from moao_amd import fits
+    from oracle import intersample as oi
+    from oracle import lapack_path as lp
+    from oracle import matcov as om
+    from oracle.check import compare_matrices
+    from oracle.tomo import Tomo as OTomo
+    d = str(tmp_path)
+    s, atms, t, n, m, dm, Dx, idx, a2f, otf, N = _psf_system(d)
+    chk = os.path.join(d, "check")
+    os.makedirs(chk)
+    o = OTomo.from_files(s, atms[0])
+    o.set_target(0)
+    Co, Bo, To = om.cmm(o), om.ctm(o), om.ctt(o)
+    Ro = Bo.copy(order="F")
+    assert lp.reconstructor(Co.copy(order="F"), Ro) == 0
+    Ceo, Cvo = lp.cee_cvv(Co.copy(order="F"), To.copy(order="F"), Bo.copy(order="F"), Ro,
+                          np.asfortranarray(Dx))
+    Po = oi.process(Cvo, idx, a2f, otf, N, dm.ndiam)
+    refs = {"Cmm": Co, "Ctm": Bo, "R": Ro, "Ctt": To, "Cee": Ceo, "Cvv": Cvo, "psf": Po}
+    for k, v in refs.items():
+        fits.write_image(os.path.join(chk, k + ".fits"), np.asfortranarray(v), v.shape[0],
+                         v.shape[1])
+    exe = os.path.join(ROOT, "moao_amd", "_lib", "check_moao")
+    out = os.path.join(d, "out")
+    os.makedirs(out)
+    args = [exe, "--sys_path", d, "--atm_path", d, "--sys_check", chk, "--compare_all", "1",
+            "--write_all", "1", "--out_path", out]
+    r = subprocess.run(args + ["--accuracy", "1e300"], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = {ln.split(":")[0].strip(): [float(x) for x in ln.split(":")[1].split()]
+             for ln in r.stdout.splitlines() if re.match(r"^(F norm|max err)\s*:", ln)}
+    names = ["Cmm", "Ctm", "Ctt", "R", "Cee", "Cvv", "psf"]
+    for k, (mx, fr) in zip(names, zip(lines["max err"], lines["F norm"])):
+        got, _ = fits.read_image(os.path.join(out, k + ".fits"))
+        omx, ofr = compare_matrices(got, refs[k], refs[k].shape[0], refs[k].shape[1], "N", "T")
+        print(f"    {k}: max {mx:.3e} (oracle metric {omx:.3e}), F {fr:.3e} ({ofr:.3e})")
+        # the table prints 7 significant digits
+        assert abs(mx - omx) <= 2e-6 * omx + 1e-300
+        assert abs(fr - ofr) <= 2e-6 * ofr + 1e-300
+    # the stages themselves: Cmm lower part and Ctt at the matcov tolerance
+    assert lines["F norm"][0] <= 1e-12 * np.linalg.norm(Co)
+    # a wrong reference PSF fails the gate
+    fits.write_image(os.path.join(chk, "psf.fits"), np.asfortranarray(2.0 * Po), N, N)
+    r = subprocess.run(args + ["--accuracy", "0.1"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and "bad accuracy" in r.stderr
+
+
+@pytest.mark.gpu
+def test_long_exposure_psf(tmp_path):
+    """maxobs = 2: the long-exposure PSF written per target is the mean of the two
+    short-exposure PSFs of the run (the reference's TODO at chameleon/moao_chameleon.cpp:
+    324-345), bit for bit."""
+    from moao_amd import fits
+    d = str(tmp_path)
+    *_, N = _psf_system(d, nprof=3)
+    r = subprocess.run([EXE, "--sys_path", d, "--atm_path", d, "--maxrefine", "1", "--maxobs",
+                        "2", "--psf", "1", "--out", d], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    p0, _ = fits.read_image(os.path.join(d, "psf_oloop0_iloop0_gal0.fits"))
+    p1, _ = fits.read_image(os.path.join(d, "psf_oloop0_iloop1_gal0.fits"))
+    lp_, dims = fits.read_image(os.path.join(d, "psf_nbrefine1_maxobs2_gal0.fits"))
+    assert dims == (N, N)
+    assert not np.array_equal(p0, p1)
+    assert np.array_equal(lp_, (p0 + p1) * 0.5)

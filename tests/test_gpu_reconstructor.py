@@ -247,3 +247,49 @@ def test_nan_pivot_reports_info():
     # every column; the blocks before it are checked.
     S = np.tril(C) + np.tril(C, -1).T
     _partial_factor_ok(Cg, S, 144, 128)
+
+
+def test_solve_graph_replays_match_eager():
+    """The FP64 solve recorded as a graph (moao_hip_solve_graph_*) and replayed three times on
+    refilled buffers -- two different systems of the same shape, eager solves and copies of the
+    program's own in between -- gives R and the factor bit-identical to the eager solve each
+    time."""
+    import ctypes
+    from moao_amd.device import DeviceArray, synchronize
+    from moao_amd.lib import check, lib
+    from moao_amd.reconstructor import reconstructor_dev
+    print("[recon] solve graph", flush=True)
+    rng = np.random.default_rng(31)
+    n, m = 2100, 333
+    probs = []
+    for _ in range(2):
+        G = rng.standard_normal((n, n))
+        C = np.asfortranarray(np.tril(G @ G.T / n + np.eye(n)))
+        probs.append((C, np.asfortranarray(rng.standard_normal((m, n)))))
+    eager = []
+    for C, B in probs:
+        dC, dB = DeviceArray.from_host(C), DeviceArray.from_host(B)
+        assert reconstructor_dev(n, m, dC.ptr, n, dB.ptr, m, 1) == 0
+        eager.append((dC.to_host(), dB.to_host()))
+    src = [(DeviceArray.from_host(C), DeviceArray.from_host(B)) for C, B in probs]
+    dC, dB = DeviceArray((n, n)), DeviceArray((m, n))
+    info = DeviceArray((1,), np.int32)
+    g = ctypes.c_void_p()
+    vp = ctypes.c_void_p
+    check(lib().moao_hip_solve_graph_create(n, m, vp(dC.ptr), n, vp(dB.ptr), m, 1,
+                                            vp(info.ptr), ctypes.byref(g)),
+          "moao_hip_solve_graph_create")
+    try:
+        for rep, k in enumerate([0, 1, 0]):
+            dC.copy_from(src[k][0])
+            dB.copy_from(src[k][1])
+            check(lib().moao_hip_solve_graph_launch(g, None), "moao_hip_solve_graph_launch")
+            synchronize()
+            assert int(info.to_host()[0]) == 0
+            assert np.array_equal(np.tril(dC.to_host()), np.tril(eager[k][0])), rep
+            assert np.array_equal(dB.to_host(), eager[k][1]), rep
+            # eager work of the program between replays
+            e1, e2 = DeviceArray.from_host(probs[1 - k][0]), DeviceArray.from_host(probs[1 - k][1])
+            assert reconstructor_dev(n, m, e1.ptr, n, e2.ptr, m, 1) == 0
+    finally:
+        lib().moao_hip_solve_graph_destroy(g)

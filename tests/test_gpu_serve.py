@@ -128,3 +128,31 @@ def test_failed_request_is_isolated(tmp_path):
     s, a, n, m = syst[1]
     with Server(slots=1) as srv:
         assert np.array_equal(after["R"], srv.wait(srv.submit(s, a, 0, MOAO_FP64, n_m=(n, m)))["R"])
+
+
+def test_repeated_shapes_replay_graphs(tmp_path):
+    """A stream of refine steps of one system (same shape, fresh atmospheric profiles) on one
+    slot: the first FP64 solve runs eagerly, the second is recorded as a graph (the pool handed
+    back the same block) and the later ones replay it.  Every R equals the one computed by the
+    eager entry on the same matcov inputs, bit for bit."""
+    from moao_amd.device import DeviceArray, synchronize
+    from moao_amd.reconstructor import reconstructor_dev
+    spec = systems.SystemSpec(diam=8.0, nssp=20, nlgs=4, nngs=1, lgs_ring_diam_as=60.0,
+                              ngs_field_diam_as=60.0, nlayer=4, r0=0.15, seed=21)
+    s, atms = systems.make_system(str(tmp_path), spec, nprof=5)
+    t0 = Tomo(s, atms[0])
+    n, m = t0.nmeas(), t0.nmeasts()
+    print(f"[serve-graph] n={n} m={m}, {len(atms)} profiles", flush=True)
+    with Server(slots=1) as srv:
+        res = [srv.wait(srv.submit(s, a, 0, MOAO_FP64, n_m=(n, m))) for a in atms]
+        st = srv.stats()
+    print(f"    captures={st['graph_captures']} replays={st['graph_replays']}", flush=True)
+    assert st["graph_captures"] + st["graph_replays"] <= len(atms) - 1
+    assert st["graph_captures"] >= 1 and st["graph_replays"] >= 1
+    for a, r in zip(atms, res):
+        t = Tomo(s, a)
+        t.set_target(0)
+        dC, dB = DeviceArray.from_host(gm.cmm(t)), DeviceArray.from_host(gm.ctm(t))
+        assert reconstructor_dev(n, m, dC.ptr, n, dB.ptr, m, 1) == 0
+        synchronize()
+        assert np.array_equal(r["R"], dB.to_host())
