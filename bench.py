@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload elt74]
                     [--modes fp64,mixed,bf16] [--serve 256] [--serve-modes fp64,bf16]
+                    [--serve-mixed 128] [--obs-loop 1]
                     [--slots 4] [--no-cpu]
 
 One step = one R-solve of the workload: restore the resident inputs (device-to-device copies of
@@ -16,10 +17,14 @@ latency         p50 / p99 of the per-step wall time (each step synchronised)
 roofline        FP64 MFMA GEMM engine: algorithmic GEMM flops / summed GEMM kernel time
                 (HIP events on the launch stream) vs the 78.6 TFLOP/s FP64 peak
 serving         a Poisson stream of uneven systems (n up to 40k) through the serving engine:
-                throughput and p50/p99 latency from submit to R on the host
+                throughput and p50/p99 latency from submit to R on the host, FP64 (256 requests)
+                and split-BF16 mixed (128), with the slots' graph captures and replays
+observation_loop  the refine/observation loop of the `moao` driver on the n = 18,688 system with
+                8 directions (4 refine x 16 observation steps, Cee/Cvv, batched PSFs, long
+                exposure), and the PSF Strehl with FP32 and FP64 transforms
 cpu_baseline    the oracle's LAPACK path (scipy dpotrf + 2 dtrsm, OpenBLAS) on the same full
-                matrices, all host cores; plus the oracle's matcov on a bounded column sample,
-                one thread and all host threads
+                matrices, all host cores; the oracle's matcov on a bounded column sample, one
+                thread and all host threads; the oracle's FP64 PSF of the loop's last step
 """
 from __future__ import annotations
 
@@ -248,6 +253,72 @@ def stages(tomo, n, m, dCmm0, dCtm0, dR, ndir=8, N=2048, dm_nssp=79, reps=2):
     return out
 
 
+def observation_loop(maxrefine=4, maxobs=16, ndir=8, N=2048, dm_nssp=79):
+    """The reference's refine / observation loop (lapack/moao_lapack.cpp:95-188) through the
+    `moao` driver on the n = 18,688 system (40x40 subapertures, 6 LGS + 2 NGS, 10 layers) with
+    8 truth directions: maxrefine R-solves (all directions at once) and maxobs observation steps
+    per refine (profiles prof{r*maxobs-1} and prof{r*maxobs+o}), each step Cmm, then per
+    direction Ctm, Ctt and Cee/Cvv (80x80 DM, Gaussian Dx), the 8 PSFs (N x N, FP32 FFTs) in
+    one batched call and the long-exposure PSF of each direction.  Wall time of the whole run,
+    the driver's own stage times, and the Strehl of the last step's direction-0 PSF with FP32
+    and FP64 transforms."""
+    import ctypes
+    import dataclasses
+    import subprocess
+    from moao_amd import fits, psf_inputs, systems
+    from moao_amd.lib import check, lib
+    d = tempfile.mkdtemp(prefix="moao_obs_")
+    spec = dataclasses.replace(systems.WORKLOADS["elt"], ntarget=ndir)
+    s, atms = systems.make_system(d, spec, nprof=maxrefine * maxobs + 1)
+    from moao_amd.tomo import Tomo
+    t = Tomo(s, atms[0])
+    n, m = t.nmeas(), t.nmeasts()
+    dm = psf_inputs.make_dm(dm_nssp)
+    nact = dm.nact
+    fits.write_image(os.path.join(d, "Dx.fits"), systems.dx_matrix(nact, m, n), nact, m)
+    idx = psf_inputs.make_idx(dm)
+    a2f = psf_inputs.make_abs2fi(N, spec.diam, dm)
+    otf = psf_inputs.make_otf_tel(N, spec.diam, spec.obs)
+    fits.write_image(os.path.join(d, "idx.fits"), idx, idx.size, 1)
+    fits.write_image(os.path.join(d, "abs2fi.fits"), a2f, N, N)
+    fits.write_image(os.path.join(d, "otftel.fits"), otf, N, N)
+    exe = os.path.join(ROOT, "moao_amd", "_lib", "moao")
+    t0 = time.perf_counter()
+    r = subprocess.run([exe, "--sys_path", d, "--atm_path", d, "--maxrefine", str(maxrefine),
+                        "--maxobs", str(maxobs), "--snapPerNight", str(maxobs), "--psf", "1",
+                        "--out", d, "--write_short", "0", "--dm_np", str(dm.ndiam)],
+                       capture_output=True, text=True)
+    wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError("moao observation loop failed: " + r.stdout[-2000:] + r.stderr[-2000:])
+    times = {}
+    for ln in r.stdout.splitlines():
+        parts = ln.split(":")
+        if len(parts) == 2 and parts[0].strip() in ("preprocess", "matcov", "R", "Cee_Cvv",
+                                                    "psf", "total"):
+            times[parts[0].strip()] = float(parts[1])
+    Cvv, _ = fits.read_image(os.path.join(d, "Cvv.fits"))
+    strehl = {}
+    for prec in (0, 1):
+        h = ctypes.c_void_p()
+        lp_ = ctypes.POINTER(ctypes.c_long)
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(lib().moao_hip_intersample_create(idx.size, dm.ndiam, N, idx.ctypes.data_as(lp_),
+                                                a2f.ctypes.data_as(dp), otf.ctypes.data_as(dp),
+                                                prec, ctypes.byref(h)), "intersample_create")
+        psf = np.empty(N * N)
+        check(lib().moao_hip_intersample_process(h, Cvv.ctypes.data_as(dp),
+                                                 psf.ctypes.data_as(dp)), "intersample_process")
+        lib().moao_hip_intersample_destroy(h)
+        strehl["fp32_fft" if prec == 0 else "fp64_fft"] = float(psf.reshape(N, N)[N // 2, N // 2])
+    longexp, _ = fits.read_image(os.path.join(d, f"psf_nbrefine{maxrefine}_maxobs{maxobs}_gal0.fits"))
+    return {"n": n, "m": m, "nact": nact, "directions": ndir, "N": N, "maxrefine": maxrefine,
+            "maxobs": maxobs, "wall_s": wall, "driver_times_s": times,
+            "strehl_last_step_dir0": strehl,
+            "strehl_long_exposure_dir0": float(longexp.reshape(N, N)[N // 2, N // 2]),
+            "_inputs": (d, idx, a2f, otf, dm.ndiam)}
+
+
 def serving(nreq, mode, slots, seed=2024, util=0.8, solve_factor=1.0):
     """Request stream of uneven systems with Poisson arrivals through the serving engine.
 
@@ -337,9 +408,14 @@ def main():
     ap.add_argument("--max-iter", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-matcov", action="store_true")
+    ap.add_argument("--obs-loop", type=int, default=1,
+                    help="run the refine/observation loop leg (n=18688, 8 directions, PSFs)")
     ap.add_argument("--serve", type=int, default=256,
                     help="requests in the serving leg (0 = skip it)")
     ap.add_argument("--serve-modes", default="fp64,bf16")
+    ap.add_argument("--serve-mixed", type=int, default=128,
+                    help="requests in the mixed-precision serving legs (their solves take "
+                         "about three times longer; 0 = as many as --serve)")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--no-stages", action="store_true",
                     help="skip the Cee/Cvv and PSF stage measurements")
@@ -532,11 +608,20 @@ def main():
                 fac = results[sm]["total"] / results["fp64"]["total"]
             elif sm != "fp64":
                 fac = 4.0
-            sv = serving(args.serve, sm, args.slots, solve_factor=fac)
+            nreq = args.serve if sm == "fp64" or args.serve_mixed <= 0 else args.serve_mixed
+            sv = serving(nreq, sm, args.slots, solve_factor=fac)
             print(f"[bench] serving {sm}: {sv['throughput_solves_per_s']:.2f} solves/s, "
                   f"p50 {sv['latency_ms']['p50']:.0f} ms, p99 {sv['latency_ms']['p99']:.0f} ms",
                   file=sys.stderr, flush=True)
             out["serving"].append(sv)
+    obs = None
+    if args.obs_loop:
+        print("[bench] observation loop (n=18688, 4 refine x 16 obs, 8 directions)",
+              file=sys.stderr, flush=True)
+        obs = observation_loop()
+        out["observation_loop"] = obs
+        print(f"[bench] observation loop: {obs['wall_s']:.1f} s, Strehl "
+              f"{obs['strehl_last_step_dir0']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         print("[bench] cpu baseline: full oracle solve", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(dCmm0, dCtm0, n, m)
@@ -551,6 +636,25 @@ def main():
                 "serial_matcov": cm["serial"]["full_s"] + solve_s}
             if "fp64" in results:
                 out["end_to_end_s"] = {"gpu": t_matcov + results["fp64"]["total"] / args.steps}
+        if obs is not None:
+            # the last observation step's direction-0 PSF on the host: the oracle's FP64
+            # restatement of intersample_process (numpy FFTs), timed, and the Strehl difference
+            from moao_amd import fits
+            from oracle import intersample as oi
+            d, idx, a2f, otf, ndiam = obs.pop("_inputs")
+            Cvv, _ = fits.read_image(os.path.join(d, "Cvv.fits"))
+            t0 = time.perf_counter()
+            P = oi.process(Cvv, idx, a2f, otf, obs["N"], ndiam)
+            dt = time.perf_counter() - t0
+            st = float(P[obs["N"] // 2, obs["N"] // 2])
+            out["cpu_baseline"]["psf"] = {
+                "s_per_direction": dt, "strehl_dir0": st,
+                "sample": f"oracle numpy FP64 intersample of the last step's direction-0 Cvv, "
+                          f"N={obs['N']}"}
+            obs["strehl_delta_vs_cpu_fp64"] = {
+                k: v - st for k, v in obs["strehl_last_step_dir0"].items()}
+    if obs is not None:
+        obs.pop("_inputs", None)
     print(json.dumps(out))
 
 

@@ -182,6 +182,33 @@ int moao_hip_sreconstructor(long nmeas, long nmeasts, float* Cmm, long ldCmm, fl
 int moao_hip_sreconstructor_dev(long nmeas, long nmeasts, float* dCmm, long ldCmm, float* dCtm,
                                 long ldCtm, long ngal, void* stream);
 
+/* Tile descriptors and sequences: MOAO_CHAMELEON::reconstructor<T>(Cmm, Ctm, sequence, check,
+ * sync) (chameleon/compute/reconstructor.hpp:15-20, reconstructor.cpp:31-65).  A descriptor is
+ * a device matrix in tile layout: tile (i, j) of mb x nb elements (the reference's ts),
+ * column-major with leading dimension mb, at element offset (i + j*mt)*mb*nb with
+ * mt = ceil(lm/mb), edge tiles padded to mb x nb (MORSE_Lapack_to_Tile's role is
+ * moao_hip_lapack_to_tile).  A sequence is a stream plus a status: calls return once queued,
+ * moao_hip_sequence_wait (MORSE_Sequence_Wait) blocks and returns the first non-zero status
+ * (a dpotrf info > 0 or MOAO_ERR_*).  reconstructor_tile_async: Cmm (n x n) -> its factor,
+ * Ctm (m x n) -> R, FP64 (dtype 0) or FP32 storage (dtype 1, the Chameleon production
+ * precision); sync != 0 waits after the solve; check is ignored (a TODO in the reference). */
+typedef struct moao_hip_desc {
+    int dtype;     /* 0 = double, 1 = float */
+    void* mat;     /* device memory in tile layout */
+    int mb, nb;    /* tile size */
+    long lm, ln;   /* matrix rows, columns */
+} moao_hip_desc;
+typedef struct moao_hip_sequence moao_hip_sequence;
+int moao_hip_sequence_create(moao_hip_sequence** seq);
+int moao_hip_sequence_wait(moao_hip_sequence* seq);
+void moao_hip_sequence_destroy(moao_hip_sequence* seq);
+int moao_hip_lapack_to_tile(const void* dA, long lda, moao_hip_desc* desc,
+                            moao_hip_sequence* seq);
+int moao_hip_tile_to_lapack(const moao_hip_desc* desc, void* dA, long lda,
+                            moao_hip_sequence* seq);
+int moao_hip_reconstructor_tile_async(const moao_hip_desc* Cmm, moao_hip_desc* Ctm,
+                                      moao_hip_sequence* seq, int check, int sync);
+
 /* Mixed precision (no reference counterpart; same result contract as the FP64 entry to a
  * stated tolerance).  Cmm is read only (lower triangle) and preserved; the Ctm block (one
  * target) is overwritten by R.  If `lowrank` describes the system Cmm was generated from, the

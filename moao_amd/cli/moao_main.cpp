@@ -16,7 +16,9 @@
 // step are computed in one batched call, and the long-exposure PSF of each target (the mean of
 // its short-exposure PSFs over every refine and observation step, which the reference leaves
 // as a TODO at chameleon/moao_chameleon.cpp:324-345) is written as
-// psf_nbrefine{maxrefine}_maxobs{maxobs}_gal{g}.fits.
+// psf_nbrefine{maxrefine}_maxobs{maxobs}_gal{g}.fits (--write_short 0 skips the short-exposure
+// files).  The DM has Nssp(truth) + 1 actuators across, as in the reference, unless --dm_np N
+// says otherwise (a DM finer than the truth sensor's grid).
 //
 // Check mode (lapack/check_moao_lapack.cpp, also selected by running the binary as
 // check_moao): --sys_check DIR [--compare_all 0|1] [--write_all 0|1] [--accuracy 1e-7]
@@ -159,7 +161,8 @@ struct Args {
                                           {"verbose", "0"}, {"mode", "fp64"},
                                           {"tol", "1e-11"}, {"psf", "0"}, {"out", ""},
                                           {"compare_all", "0"}, {"write_all", "0"},
-                                          {"accuracy", "1.e-7"}, {"out_path", ""}};
+                                          {"accuracy", "1.e-7"}, {"out_path", ""},
+                                          {"write_short", "1"}, {"dm_np", "0"}};
     std::string s(const char* k) { return kv[k]; }
     long i(const char* k) { return std::atol(kv[k].c_str()); }
 };
@@ -218,7 +221,8 @@ int run_check(Args& a) {
     auto dCmm = zeroed((size_t)n * n), dR = zeroed((size_t)m * n), dCtm = zeroed((size_t)m * n),
          dCtt = zeroed((size_t)m * m), dCee = zeroed((size_t)m * m),
          dCvv = zeroed((size_t)nact * nact), dDx = zeroed((size_t)nact * m),
-         dTmp = zeroed((size_t)nact * m), dRef = zeroed((size_t)std::max(n * n, std::max(nact * nact, m * n)));
+         dTmp = zeroed((size_t)nact * m),
+         dRef = zeroed((size_t)std::max(std::max(n * n, m * n), std::max(nact * nact, N * N)));
     std::map<std::string, std::pair<double, double>> err;  // stage -> (max, frobenius)
     auto compare = [&](const char* name, const double* d, long nr, long nc, bool always) {
         if (!outp.empty() && write_all) {
@@ -384,8 +388,11 @@ int main(int argc, char** argv) {
         std::vector<double> ns(dims[0]);
         moao_tomo_field(tomo, "Nssp", ns.data(), dims[0]);
         std::vector<long> idx(idxd.begin(), idxd.end());
-        check(moao_hip_intersample_create(nidx, (long)ns[dims[0] - 1] + 1, nx, idx.data(),
-                                          a2f.data(), otf.data(), 0, &isample),
+        // actuators across the DM: the reference's rule Nssp(truth) + 1 (moao_lapack.cpp:64)
+        // unless --dm_np gives the DM explicitly
+        const long np = a.i("dm_np") > 0 ? a.i("dm_np") : (long)ns[dims[0] - 1] + 1;
+        check(moao_hip_intersample_create(nidx, np, nx, idx.data(), a2f.data(), otf.data(), 0,
+                                          &isample),
               "intersample");
         N = moao_hip_intersample_n(isample);
     }
@@ -469,7 +476,7 @@ int main(int argc, char** argv) {
                 check(moao_hip_memcpy_d2h(psf.data(), dPsf.p, psf.size() * 8), "psf");
                 t_psf.stop();
                 for (size_t e = 0; e < psf.size(); ++e) cumul[e] += psf[e];
-                if (!out.empty())
+                if (!out.empty() && a.i("write_short"))
                     for (long g = 0; g < nTarget; ++g)
                         fits_write(out + "/psf_oloop" + std::to_string(r) + "_iloop" +
                                        std::to_string(o) + "_gal" + std::to_string(g) + ".fits",

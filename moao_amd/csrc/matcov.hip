@@ -233,8 +233,12 @@ __global__ void __launch_bounds__(kWaves * 64) pair_kernel(MatcovArgs a) {
         const int n = (ce.x >> 1) & 0xff, p2 = ce.x >> 9;
         const int xc = ce.y, yc = ce.y + a.nsub[n];
         // type 1 writes only row >= column: nothing of this pair when its largest row is above
-        // its smallest column
-        if (a.type_mat == 1 && yr < xc) continue;
+        // its smallest column; in a sensor's diagonal block a pair may need only some of its
+        // four elements, and only their structure functions are evaluated
+        const bool lo = a.type_mat == 1;
+        const bool nxx = !lo || xr >= xc, nyy = !lo || yr >= yc;
+        const bool nxy = !lo || xr >= yc, nyx = !lo || yr >= xc;
+        if (!nyx) continue;  // yx has the largest row and the smallest column
         const double kk = kLambda2 / (a.sspsz[m] * a.sspsz[n]);
         double cxx = 0.0, cxy = 0.0, cyx = 0.0, cyy = 0.0;
         for (int l = 0; l < a.Nlayer; ++l) {
@@ -248,24 +252,35 @@ __global__ void __launch_bounds__(kWaves * 64) pair_kernel(MatcovArgs a) {
                 const double lk = a.lk[l], amp = a.amp[l];
                 const double ac = s1 - s2, ad = s1 + s2, bc = -ad, bd = -ac;
                 double sxx, syy, sxy, syx;
+                sxx = syy = sxy = 0.0;
                 if (ac == 0.0) {
-                    const double d0 = vk_dphi(du, dv, lk, amp);
-                    sxx = -d0 + vk_dphi(du + ad, dv, lk, amp) + vk_dphi(du + bc, dv, lk, amp) - d0;
-                    syy = -d0 + vk_dphi(du, dv + ad, lk, amp) + vk_dphi(du, dv + bc, lk, amp) - d0;
+                    if (nxx || nyy) {
+                        const double d0 = vk_dphi(du, dv, lk, amp);
+                        if (nxx)
+                            sxx = -d0 + vk_dphi(du + ad, dv, lk, amp) +
+                                  vk_dphi(du + bc, dv, lk, amp) - d0;
+                        if (nyy)
+                            syy = -d0 + vk_dphi(du, dv + ad, lk, amp) +
+                                  vk_dphi(du, dv + bc, lk, amp) - d0;
+                    }
                     // s1 == s2: the corners of xy and yx are the same four arguments
                     const double q1 = vk_dphi(du + s1, dv - s2, lk, amp);
                     const double q2 = vk_dphi(du + s1, dv + s2, lk, amp);
                     const double q3 = vk_dphi(du - s1, dv - s2, lk, amp);
                     const double q4 = vk_dphi(du - s1, dv + s2, lk, amp);
-                    sxy = -q1 + q2 + q3 - q4;
+                    if (nxy) sxy = -q1 + q2 + q3 - q4;
                     syx = -q4 + q2 + q3 - q1;
                 } else {
-                    sxx = -vk_dphi(du + ac, dv, lk, amp) + vk_dphi(du + ad, dv, lk, amp) +
-                          vk_dphi(du + bc, dv, lk, amp) - vk_dphi(du + bd, dv, lk, amp);
-                    syy = -vk_dphi(du, dv + ac, lk, amp) + vk_dphi(du, dv + ad, lk, amp) +
-                          vk_dphi(du, dv + bc, lk, amp) - vk_dphi(du, dv + bd, lk, amp);
-                    sxy = -vk_dphi(du + s1, dv - s2, lk, amp) + vk_dphi(du + s1, dv + s2, lk, amp) +
-                          vk_dphi(du - s1, dv - s2, lk, amp) - vk_dphi(du - s1, dv + s2, lk, amp);
+                    if (nxx)
+                        sxx = -vk_dphi(du + ac, dv, lk, amp) + vk_dphi(du + ad, dv, lk, amp) +
+                              vk_dphi(du + bc, dv, lk, amp) - vk_dphi(du + bd, dv, lk, amp);
+                    if (nyy)
+                        syy = -vk_dphi(du, dv + ac, lk, amp) + vk_dphi(du, dv + ad, lk, amp) +
+                              vk_dphi(du, dv + bc, lk, amp) - vk_dphi(du, dv + bd, lk, amp);
+                    if (nxy)
+                        sxy = -vk_dphi(du + s1, dv - s2, lk, amp) +
+                              vk_dphi(du + s1, dv + s2, lk, amp) +
+                              vk_dphi(du - s1, dv - s2, lk, amp) - vk_dphi(du - s1, dv + s2, lk, amp);
                     syx = -vk_dphi(du - s2, dv + s1, lk, amp) + vk_dphi(du + s2, dv + s1, lk, amp) +
                           vk_dphi(du - s2, dv - s1, lk, amp) - vk_dphi(du + s2, dv - s1, lk, amp);
                 }
@@ -290,13 +305,12 @@ __global__ void __launch_bounds__(kWaves * 64) pair_kernel(MatcovArgs a) {
                 cyy += a.nNGS[m - a.nlgs];
             }
         }
-        const bool lo = a.type_mat == 1;
         double* ox = a.out + (long)xc * a.lda;
         double* oy = a.out + (long)yc * a.lda;
-        if (!lo || xr >= xc) ox[xr] = cxx;
-        if (!lo || yr >= xc) ox[yr] = cyx;
-        if (!lo || xr >= yc) oy[xr] = cxy;
-        if (!lo || yr >= yc) oy[yr] = cyy;
+        if (nxx) ox[xr] = cxx;
+        ox[yr] = cyx;
+        if (nxy) oy[xr] = cxy;
+        if (nyy) oy[yr] = cyy;
     }
 }
 

@@ -77,6 +77,12 @@ class IsampleStruct(ctypes.Structure):
                 ("dev", ctypes.c_void_p)]
 
 
+class TileDesc(ctypes.Structure):
+    """ctypes mirror of moao_hip_desc."""
+    _fields_ = [("dtype", ctypes.c_int), ("mat", ctypes.c_void_p), ("mb", ctypes.c_int),
+                ("nb", ctypes.c_int), ("lm", ctypes.c_long), ("ln", ctypes.c_long)]
+
+
 class ServeCounters(ctypes.Structure):
     """ctypes mirror of moao_serve_counters."""
     _fields_ = [("queued", ctypes.c_long), ("running", ctypes.c_long),
@@ -155,6 +161,17 @@ _SIGS = {
                                                    ctypes.POINTER(ctypes.c_void_p)]),
     "moao_hip_solve_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "moao_hip_solve_graph_destroy": (None, [ctypes.c_void_p]),
+    "moao_hip_sequence_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "moao_hip_sequence_wait": (ctypes.c_int, [ctypes.c_void_p]),
+    "moao_hip_sequence_destroy": (None, [ctypes.c_void_p]),
+    "moao_hip_lapack_to_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_long,
+                                               ctypes.POINTER(TileDesc), ctypes.c_void_p]),
+    "moao_hip_tile_to_lapack": (ctypes.c_int, [ctypes.POINTER(TileDesc), ctypes.c_void_p,
+                                               ctypes.c_long, ctypes.c_void_p]),
+    "moao_hip_reconstructor_tile_async": (ctypes.c_int, [ctypes.POINTER(TileDesc),
+                                                         ctypes.POINTER(TileDesc),
+                                                         ctypes.c_void_p, ctypes.c_int,
+                                                         ctypes.c_int]),
     "moao_hip_reconstructor_mixed_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long,
                                                         ctypes.c_void_p, ctypes.c_long,
                                                         ctypes.c_void_p, ctypes.c_long,

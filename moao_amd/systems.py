@@ -47,6 +47,8 @@ class SystemSpec:
     fixed_h: List[float] = field(default_factory=list)
     lgs_cst: float = 1.0
     seed: int = 1234
+    ntarget: int = 1                      # truth directions (the first on axis)
+    target_field_diam_as: float = 300.0   # the others uniform in the disk of this field
 
 
 WORKLOADS = {
@@ -82,11 +84,18 @@ def write_sys(path: str, spec: SystemSpec, rng: np.random.Generator):
         ay.append(r * math.sin(a))
     ax.append(0.0)
     ay.append(0.0)
+    tx, ty = [0.0], [0.0]
+    for _ in range(spec.ntarget - 1):
+        r = spec.target_field_diam_as / 2.0 * math.sqrt(rng.uniform())
+        a = rng.uniform(0, 2 * math.pi)
+        tx.append(r * math.cos(a))
+        ty.append(r * math.sin(a))
     gs_alt = [1.0 / spec.lgs_alt] * spec.nlgs + [0.0] * (spec.nngs + 1)
     gs_type = [2] * spec.nlgs + [1] * (spec.nngs + 1)
     rows = [
         ("tel diam in meters", spec.diam), ("cent obs", spec.obs), ("Tframe s", 0.004),
-        ("N WFS", nW), ("nlgs", spec.nlgs), ("nTarget", 1), ("Nssp n subaps per wfs", spec.nssp),
+        ("N WFS", nW), ("nlgs", spec.nlgs), ("nTarget", spec.ntarget),
+        ("Nssp n subaps per wfs", spec.nssp),
         ("GsAlt", gs_alt), ("type", gs_type), ("alphaX in arcsec", ax), ("alphaY in arcsec", ay),
         ("XPup", [0.0] * nW), ("YPup", [0.0] * nW), ("thetaML", [0.0] * nW),
         ("thetaCam", [0.0] * nW), ("sensibilite", [1.0] * nW), ("tracking", [0.0] * 3),
@@ -97,7 +106,7 @@ def write_sys(path: str, spec: SystemSpec, rng: np.random.Generator):
         ("transmission for LGS", 0.382), ("atmosphere transmission", 0.84),
         ("Read Out Noise (nb of e-)", 3), ("lgs_cst", spec.lgs_cst), ("spot_width arcsec", 1.0),
         ("lgs_alt (m)", spec.lgs_alt), ("lgs_depth (m)", 5000.0),
-        ("targetX  in arcsec", [0.0]), ("targetY  in arcsec", [0.0]),
+        ("targetX  in arcsec", tx), ("targetY  in arcsec", ty),
     ]
     with open(path, "w") as fh:
         for name, v in rows:

@@ -123,11 +123,15 @@ def test_check_mode_table_and_gate(tmp_path):
     assert lp.reconstructor(Co.copy(order="F"), Ro) == 0
     Ceo, Cvo = lp.cee_cvv(Co.copy(order="F"), To.copy(order="F"), Bo.copy(order="F"), Ro,
                           np.asfortranarray(Dx))
-    Po = oi.process(Cvo, idx, a2f, otf, N, dm.ndiam)
-    refs = {"Cmm": Co, "Ctm": Bo, "R": Ro, "Ctt": To, "Cee": Ceo, "Cvv": Cvo, "psf": Po}
-    for k, v in refs.items():
-        fits.write_image(os.path.join(chk, k + ".fits"), np.asfortranarray(v), v.shape[0],
-                         v.shape[1])
+    Po = oi.process(Cvo.flatten(order="F"), idx, a2f, otf, N, dm.ndiam)
+    # each reference flat in the library's memory order: column-major matrices; the PSF in
+    # the layout of the reference's dphi (what oracle.intersample.process returns C-ordered)
+    refs = {k: (np.asarray(v).flatten(order="F"), v.shape[0], v.shape[1])
+            for k, v in {"Cmm": Co, "Ctm": Bo, "R": Ro, "Ctt": To, "Cee": Ceo,
+                         "Cvv": Cvo}.items()}
+    refs["psf"] = (Po.flatten(), N, N)
+    for k, (v, nr, nc) in refs.items():
+        fits.write_image(os.path.join(chk, k + ".fits"), v, nr, nc)
     exe = os.path.join(ROOT, "moao_amd", "_lib", "check_moao")
     out = os.path.join(d, "out")
     os.makedirs(out)
@@ -142,7 +146,8 @@ def test_check_mode_table_and_gate(tmp_path):
     names = ["Cmm", "Ctm", "Ctt", "R", "Cee", "Cvv", "psf"]
     for k, (mx, fr) in zip(names, zip(lines["max err"], lines["F norm"])):
         got, _ = fits.read_image(os.path.join(out, k + ".fits"))
-        omx, ofr = compare_matrices(got, refs[k], refs[k].shape[0], refs[k].shape[1], "N", "T")
+        v, nr, nc = refs[k]
+        omx, ofr = compare_matrices(got, v, nr, nc, "N", "T")
         print(f"    {k}: max {mx:.3e} (oracle metric {omx:.3e}), F {fr:.3e} ({ofr:.3e})")
         # the table prints 7 significant digits
         assert abs(mx - omx) <= 2e-6 * omx + 1e-300
@@ -150,7 +155,7 @@ def test_check_mode_table_and_gate(tmp_path):
     # the stages themselves: Cmm lower part and Ctt at the matcov tolerance
     assert lines["F norm"][0] <= 1e-12 * np.linalg.norm(Co)
     # a wrong reference PSF fails the gate
-    fits.write_image(os.path.join(chk, "psf.fits"), np.asfortranarray(2.0 * Po), N, N)
+    fits.write_image(os.path.join(chk, "psf.fits"), 2.0 * refs["psf"][0], N, N)
     r = subprocess.run(args + ["--accuracy", "0.1"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 1 and "bad accuracy" in r.stderr
 

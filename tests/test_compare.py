@@ -1,8 +1,10 @@
 """The library's compareMatrices (moao_compare_matrices on the host, moao_hip_compare_matrices_dev
 on device buffers) against the oracle's restatement (oracle/check.py, itself pinned to the
 hand-built reference's compareMatrices): every uplo / trans rule, zeros in A (the |B| branch),
-non-square shapes.  max_err is order-free and must match exactly; the Frobenius sums are
-accumulated in different orders, so they match to 1e-13 relative."""
+non-square shapes.  max_err is order-free and must match exactly; the Frobenius sums of squares
+are accumulated in different orders (sequential on the host, per-workgroup partials on the
+device), so they agree within the n u bound of a sum of n positive terms: 1e-13 relative for
+the small cases, 1e-9 for the 7.5M-entry one."""
 import ctypes
 
 import numpy as np
@@ -50,4 +52,4 @@ def test_device_compare_matches_host(nr, nc, uplo, trans):
     lib().moao_compare_matrices(A.ctypes.data_as(dp), B.ctypes.data_as(dp), nr, nc,
                                 uplo.encode(), trans.encode(), ctypes.byref(hmx), ctypes.byref(hfr))
     assert mx.value == hmx.value
-    assert abs(fr.value - hfr.value) <= 1e-12 * hfr.value
+    assert abs(fr.value - hfr.value) <= max(1e-13, 2e-16 * nr * nc) * hfr.value
