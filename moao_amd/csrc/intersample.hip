@@ -5,10 +5,13 @@
 //   Map[idx[i]-1] += Cvv[i]; Map /= div          map_gather: one wave per map bin over a CSR
 //                                                 list of the Cvv entries falling in it
 //                                                 (deterministic, no atomics)
-//   embed Map in the N x N grid at stride dactupix embed (after a zero fill)
-//   r2c FFT; *= abs2fi[j*N+i] (real part); c2r    full complex FFTs; the half-spectrum product
-//                                                 is extended to the full grid by the
-//                                                 Hermitian mirror the c2r transform assumes
+//   embed Map in the N x N grid at stride dactupix embed (after a zero fill), two directions
+//                                                 per complex grid (real and imaginary parts)
+//   r2c FFT; *= abs2fi[j*N+i] (real part); c2r    complex FFTs of the packed pair (each real
+//                                                 transform costs half a complex one); the two
+//                                                 spectra separated by Hermitian symmetry, the
+//                                                 half-spectrum product extended to the full
+//                                                 grid by the mirror the c2r transform assumes
 //   dphi = exp(-0.5 (norm - dphi/fact2*fact))     dphi_kernel (fftshift folded in)
 //   _eclat; tmp = otf_tel[j*N+i]*dphi[j*N+i]; c2r spec2 (mirror as above), backward FFT
 //   _eclat -> PSF                                 final_kernel (fftshift folded in)
@@ -69,86 +72,115 @@ __global__ void zero_fill(C* X, size_t n) {
     if (i < n) X[i] = C{0, 0};
 }
 
+// Two directions per complex grid: direction 2p in the real part, 2p + 1 in the imaginary part
+// (zero when the batch is odd).  Every transform of the PSF is a transform of real data, so one
+// complex transform carries two of them: the forward spectra separate by Hermitian symmetry and
+// the later spectra are real-valued and symmetric, so their inverse transforms are real.
 template <typename C>
 __global__ void embed(const double* __restrict__ Map, long nmap, C* X, int N, int ncov, int first,
-                      int step) {
+                      int step, int batch) {
     const int t = blockIdx.x * 256 + threadIdx.x;  // map index = ti * ncov + tj
-    const int b = blockIdx.y;
+    const int p = blockIdx.y;                      // direction pair
     if (t >= ncov * ncov) return;
     const int ti = t / ncov, tj = t % ncov;
     const int i = first + ti * step, j = first + tj * step;
-    X[(size_t)b * N * N + (size_t)i * N + j] = C{(decltype(C{}.x))Map[(size_t)b * nmap + t], 0};
+    using R = decltype(C{}.x);
+    const R re = (R)Map[(size_t)(2 * p) * nmap + t];
+    const R im = 2 * p + 1 < batch ? (R)Map[(size_t)(2 * p + 1) * nmap + t] : R(0);
+    X[(size_t)p * N * N + (size_t)i * N + j] = C{re, im};
 }
 
-// Y[i][j] = (Re X[i'][j'] * w[j'*N + i'], 0), (i', j') = (i, j) for j <= N/2, else the Hermitian
-// mirror ((N-i)%N, N-j): the full-grid form of the half spectrum a c2r transform reads.
+// Y[i][j] = (Ga, Gb) with G(i, j) = Re F(i', j') * w[j'*N + i'], (i', j') = (i, j) for j <= N/2,
+// else the Hermitian mirror ((N-i)%N, N-j): the full-grid form of the half spectrum a c2r
+// transform reads.  F_a = (Z(k) + conj Z(-k)) / 2 and F_b = (Z(k) - conj Z(-k)) / 2i of the
+// packed transform Z, so Re F_a = (Re Z(k) + Re Z(-k)) / 2, Re F_b = (Im Z(k) + Im Z(-k)) / 2.
 template <typename R, typename C>
 __global__ void spec_mul(const C* __restrict__ X, const R* __restrict__ w, C* __restrict__ Y,
                          int N) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int i = blockIdx.y;
-    const int b = blockIdx.z;
+    const int p = blockIdx.z;
     if (j >= N) return;
     int ii = i, jj = j;
     if (j > N / 2) {
         ii = (N - i) % N;
         jj = N - j;
     }
-    const size_t o = (size_t)b * N * N;
-    Y[o + (size_t)i * N + j] = C{X[o + (size_t)ii * N + jj].x * w[(size_t)jj * N + ii], 0};
+    const size_t o = (size_t)p * N * N;
+    const C z = X[o + (size_t)ii * N + jj];
+    const C zm = X[o + (size_t)((N - ii) % N) * N + (N - jj) % N];
+    R wk = w[(size_t)jj * N + ii];
+    // columns 0 and N/2 hold both k and -k: the weight is averaged over the pair, so the
+    // product stays even whatever the weights (a c2r transform keeps only that even part)
+    if (jj == 0 || jj == N / 2) wk = (R)0.5 * (wk + w[(size_t)jj * N + (N - ii) % N]);
+    Y[o + (size_t)i * N + j] = C{(R)0.5 * (z.x + zm.x) * wk, (R)0.5 * (z.y + zm.y) * wk};
 }
 
 // dphi = exp(-0.5 (norm - D fact / fact2)), norm = D(N/2, N/2) fact / fact2, then fftshift;
-// stored (real) in X.
+// D of direction 2p in the real part of Y, of 2p + 1 in the imaginary part; stored packed in X.
 template <typename R, typename C>
 __global__ void dphi_kernel(const C* __restrict__ Y, C* __restrict__ X, int N, double fact,
                             double fact2) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int i = blockIdx.y;
-    const int b = blockIdx.z;
+    const int p = blockIdx.z;
     if (j >= N) return;
-    const size_t o = (size_t)b * N * N;
-    const double norm = (double)Y[o + (size_t)(N / 2) * N + N / 2].x / fact2 * fact;
+    const size_t o = (size_t)p * N * N;
+    const C c = Y[o + (size_t)(N / 2) * N + N / 2];
     const int si = (i + N / 2) % N, sj = (j + N / 2) % N;  // value landing at (i, j) after shift
-    const double d = (double)Y[o + (size_t)si * N + sj].x;
-    X[o + (size_t)i * N + j] = C{(R)exp(-0.5 * (norm - d / fact2 * fact)), 0};
+    const C d = Y[o + (size_t)si * N + sj];
+    const double na = (double)c.x / fact2 * fact, nb = (double)c.y / fact2 * fact;
+    X[o + (size_t)i * N + j] = C{(R)exp(-0.5 * (na - (double)d.x / fact2 * fact)),
+                                 (R)exp(-0.5 * (nb - (double)d.y / fact2 * fact))};
 }
 
-// spec2: G[i][j] = otf[j'*N+i'] * dphi[j'*N+i'] with the same mirror for j > N/2.
+// spec2: G[i][j] = otf[j'*N+i'] * dphi[j'*N+i'] with the same mirror for j > N/2, both
+// directions of the pair.
 template <typename R, typename C>
 __global__ void spec_otf(const C* __restrict__ X, const R* __restrict__ otf, C* __restrict__ Y,
                          int N) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int i = blockIdx.y;
-    const int b = blockIdx.z;
+    const int p = blockIdx.z;
     if (j >= N) return;
     int ii = i, jj = j;
     if (j > N / 2) {
         ii = (N - i) % N;
         jj = N - j;
     }
-    const size_t o = (size_t)b * N * N;
+    const size_t o = (size_t)p * N * N;
     const size_t k = (size_t)jj * N + ii;  // flat index j'*N + i'
-    Y[o + (size_t)i * N + j] = C{otf[k] * X[o + k].x, 0};
+    const C d = X[o + k];
+    if (jj == 0 || jj == N / 2) {  // even part, as in spec_mul
+        const size_t km = (size_t)jj * N + (N - ii) % N;
+        const C dm = X[o + km];
+        Y[o + (size_t)i * N + j] = C{(R)0.5 * (otf[k] * d.x + otf[km] * dm.x),
+                                     (R)0.5 * (otf[k] * d.y + otf[km] * dm.y)};
+        return;
+    }
+    Y[o + (size_t)i * N + j] = C{otf[k] * d.x, otf[k] * d.y};
 }
 
 template <typename C>
-__global__ void final_kernel(const C* __restrict__ Y, double* __restrict__ psf, int N) {
+__global__ void final_kernel(const C* __restrict__ Y, double* __restrict__ psf, int N, int batch) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int i = blockIdx.y;
-    const int b = blockIdx.z;
+    const int p = blockIdx.z;
     if (j >= N) return;
-    const size_t o = (size_t)b * N * N;
+    const size_t o = (size_t)p * N * N;
     const int si = (i + N / 2) % N, sj = (j + N / 2) % N;
-    psf[o + (size_t)i * N + j] = (double)Y[o + (size_t)si * N + sj].x;
+    const C v = Y[o + (size_t)si * N + sj];
+    psf[(size_t)(2 * p) * N * N + (size_t)i * N + j] = (double)v.x;
+    if (2 * p + 1 < batch) psf[(size_t)(2 * p + 1) * N * N + (size_t)i * N + j] = (double)v.y;
 }
 
 template <typename R>
 int process(moao_isample* S, const double* dCvv, int batch, double* dPsf, hipStream_t s) {
     using C = typename Cplx<R>::type;
     const int N = (int)S->N, ncov = (int)S->ncov;
+    const int pairs = (batch + 1) / 2;  // complex grids: two directions each
     if (batch > S->batch_cap) {
-        const size_t cb = sizeof(C) * (size_t)N * N * batch;
+        const size_t cb = sizeof(C) * (size_t)N * N * pairs;
         if (S->X) (void)hipFree(S->X);
         if (S->Y) (void)hipFree(S->Y);
         if (S->Map) (void)hipFree(S->Map);
@@ -163,7 +195,7 @@ int process(moao_isample* S, const double* dCvv, int batch, double* dPsf, hipStr
     C* X = (C*)S->X;
     C* Y = (C*)S->Y;
     const C* tw = (const C*)S->tw;
-    const size_t tot = (size_t)N * N * batch;
+    const size_t tot = (size_t)N * N * pairs;
     {
         ProfScope ps(kProfOther, 8.0 * S->nidx * batch, s);
         hipLaunchKernelGGL(map_gather, dim3((S->nmap + 3) / 4, batch), dim3(256), 0, s, dCvv,
@@ -171,20 +203,20 @@ int process(moao_isample* S, const double* dCvv, int batch, double* dPsf, hipStr
         hipLaunchKernelGGL(zero_fill<C>, dim3((tot + 255) / 256), dim3(256), 0, s, X, tot);
         const int nelem = (ncov - 1) / 2, ncmap = N / 2 + 1, step = (int)S->dactupix;
         const int first = (int)(ncmap - nelem * S->dactupix - 1);
-        hipLaunchKernelGGL(embed<C>, dim3((ncov * ncov + 255) / 256, batch), dim3(256), 0, s,
-                           S->Map, S->nmap, X, N, ncov, first, step);
+        hipLaunchKernelGGL(embed<C>, dim3((ncov * ncov + 255) / 256, pairs), dim3(256), 0, s,
+                           S->Map, S->nmap, X, N, ncov, first, step, batch);
         MOAO_HIP_TRY(hipGetLastError());
     }
-    if (int rc = fft2d<R>(X, Y, N, batch, -1, tw, s)) return rc;
-    const dim3 pg((N + 255) / 256, N, batch);
+    if (int rc = fft2d<R>(X, Y, N, pairs, -1, tw, s)) return rc;
+    const dim3 pg((N + 255) / 256, N, pairs);
     hipLaunchKernelGGL((spec_mul<R, C>), pg, dim3(256), 0, s, X, (const R*)S->a2f, Y, N);
-    if (int rc = fft2d<R>(Y, X, N, batch, +1, tw, s)) return rc;
+    if (int rc = fft2d<R>(Y, X, N, pairs, +1, tw, s)) return rc;
     const double fact = 2 * std::pow(2 * 3.14159 / S->lambda, 2);
     const double fact2 = (double)N * N * S->dactupix * S->dactupix;
     hipLaunchKernelGGL((dphi_kernel<R, C>), pg, dim3(256), 0, s, Y, X, N, fact, fact2);
     hipLaunchKernelGGL((spec_otf<R, C>), pg, dim3(256), 0, s, X, (const R*)S->otf, Y, N);
-    if (int rc = fft2d<R>(Y, X, N, batch, +1, tw, s)) return rc;
-    hipLaunchKernelGGL(final_kernel<C>, pg, dim3(256), 0, s, Y, dPsf, N);
+    if (int rc = fft2d<R>(Y, X, N, pairs, +1, tw, s)) return rc;
+    hipLaunchKernelGGL(final_kernel<C>, pg, dim3(256), 0, s, Y, dPsf, N, batch);
     MOAO_HIP_TRY(hipGetLastError());
     return MOAO_OK;
 }
