@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload elt74]
                     [--modes fp64,mixed,bf16] [--serve 256] [--serve-modes fp64,bf16]
-                    [--serve-mixed 128] [--obs-loop 1]
+                    [--serve-mixed 128] [--obs-loop 1] [--obs-prepare DIR]
                     [--slots 4] [--no-cpu]
 
 One step = one R-solve of the workload: restore the resident inputs (device-to-device copies of
@@ -253,6 +253,31 @@ def stages(tomo, n, m, dCmm0, dCtm0, dR, ndir=8, N=2048, dm_nssp=79, reps=2):
     return out
 
 
+def obs_prepare(d, maxrefine, maxobs, ndir, N, dm_nssp):
+    """Files of the observation-loop system in directory d (system, maxrefine * maxobs + 1
+    atmosphere profiles, Dx, idx, abs2fi, otftel) and the `moao` command line that runs it."""
+    import dataclasses
+    from moao_amd import fits, psf_inputs, systems
+    from moao_amd.tomo import Tomo
+    spec = dataclasses.replace(systems.WORKLOADS["elt"], ntarget=ndir)
+    s, atms = systems.make_system(d, spec, nprof=maxrefine * maxobs + 1)
+    t = Tomo(s, atms[0])
+    n, m = t.nmeas(), t.nmeasts()
+    dm = psf_inputs.make_dm(dm_nssp)
+    fits.write_image(os.path.join(d, "Dx.fits"), systems.dx_matrix(dm.nact, m, n), dm.nact, m)
+    idx = psf_inputs.make_idx(dm)
+    a2f = psf_inputs.make_abs2fi(N, spec.diam, dm)
+    otf = psf_inputs.make_otf_tel(N, spec.diam, spec.obs)
+    fits.write_image(os.path.join(d, "idx.fits"), idx, idx.size, 1)
+    fits.write_image(os.path.join(d, "abs2fi.fits"), a2f, N, N)
+    fits.write_image(os.path.join(d, "otftel.fits"), otf, N, N)
+    exe = os.path.join(ROOT, "moao_amd", "_lib", "moao")
+    cmd = [exe, "--sys_path", d, "--atm_path", d, "--maxrefine", str(maxrefine), "--maxobs",
+           str(maxobs), "--snapPerNight", str(maxobs), "--psf", "1", "--out", d,
+           "--write_short", "0", "--dm_np", str(dm.ndiam)]
+    return n, m, dm, idx, a2f, otf, cmd
+
+
 def observation_loop(maxrefine=4, maxobs=16, ndir=8, N=2048, dm_nssp=79):
     """The reference's refine / observation loop (lapack/moao_lapack.cpp:95-188) through the
     `moao` driver on the n = 18,688 system (40x40 subapertures, 6 LGS + 2 NGS, 10 layers) with
@@ -263,31 +288,14 @@ def observation_loop(maxrefine=4, maxobs=16, ndir=8, N=2048, dm_nssp=79):
     the driver's own stage times, and the Strehl of the last step's direction-0 PSF with FP32
     and FP64 transforms."""
     import ctypes
-    import dataclasses
     import subprocess
-    from moao_amd import fits, psf_inputs, systems
+    from moao_amd import fits
     from moao_amd.lib import check, lib
     d = tempfile.mkdtemp(prefix="moao_obs_")
-    spec = dataclasses.replace(systems.WORKLOADS["elt"], ntarget=ndir)
-    s, atms = systems.make_system(d, spec, nprof=maxrefine * maxobs + 1)
-    from moao_amd.tomo import Tomo
-    t = Tomo(s, atms[0])
-    n, m = t.nmeas(), t.nmeasts()
-    dm = psf_inputs.make_dm(dm_nssp)
+    n, m, dm, idx, a2f, otf, cmd = obs_prepare(d, maxrefine, maxobs, ndir, N, dm_nssp)
     nact = dm.nact
-    fits.write_image(os.path.join(d, "Dx.fits"), systems.dx_matrix(nact, m, n), nact, m)
-    idx = psf_inputs.make_idx(dm)
-    a2f = psf_inputs.make_abs2fi(N, spec.diam, dm)
-    otf = psf_inputs.make_otf_tel(N, spec.diam, spec.obs)
-    fits.write_image(os.path.join(d, "idx.fits"), idx, idx.size, 1)
-    fits.write_image(os.path.join(d, "abs2fi.fits"), a2f, N, N)
-    fits.write_image(os.path.join(d, "otftel.fits"), otf, N, N)
-    exe = os.path.join(ROOT, "moao_amd", "_lib", "moao")
     t0 = time.perf_counter()
-    r = subprocess.run([exe, "--sys_path", d, "--atm_path", d, "--maxrefine", str(maxrefine),
-                        "--maxobs", str(maxobs), "--snapPerNight", str(maxobs), "--psf", "1",
-                        "--out", d, "--write_short", "0", "--dm_np", str(dm.ndiam)],
-                       capture_output=True, text=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
     wall = time.perf_counter() - t0
     if r.returncode != 0:
         raise RuntimeError("moao observation loop failed: " + r.stdout[-2000:] + r.stderr[-2000:])
@@ -422,7 +430,17 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the bench, run 3 plain device-to-device copies of this many "
                          "MiB (the calibration kernel for HBM byte counters)")
+    ap.add_argument("--obs-prepare", metavar="DIR",
+                    help="only write the observation-loop system (--obs-refine x --obs-steps "
+                         "steps) to DIR and print the moao command that runs it (no GPU use)")
+    ap.add_argument("--obs-refine", type=int, default=4)
+    ap.add_argument("--obs-steps", type=int, default=16)
     args = ap.parse_args()
+    if args.obs_prepare:
+        os.makedirs(args.obs_prepare, exist_ok=True)
+        *_, cmd = obs_prepare(args.obs_prepare, args.obs_refine, args.obs_steps, 8, 2048, 79)
+        print(" ".join(cmd))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note, not built)"}))

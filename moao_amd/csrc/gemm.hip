@@ -35,7 +35,10 @@ namespace {
 constexpr int BM = 128, BN = 128, BK = 8;
 constexpr int PAD = 2;
 constexpr int NT = 256;
-constexpr int kStages = 3;  // direct-to-LDS pipeline depth (slices in flight + 1)
+#ifndef MOAO_GEMM_STAGES
+#define MOAO_GEMM_STAGES 3
+#endif
+constexpr int kStages = MOAO_GEMM_STAGES;  // direct-to-LDS pipeline depth (slices in flight + 1)
 
 template <typename T>
 struct Mfma;
@@ -257,6 +260,56 @@ __device__ __forceinline__ void bar() {
     asm volatile("" ::: "memory");
 }
 
+// Tile order of a launch.  The dispatcher hands workgroups to the 8 XCDs round robin by launch
+// index (index % 8), and each XCD has an L2 of its own.  The launch index L is re-mapped so
+// that the k-th run of kXcdChunk workgroups an XCD receives is kXcdChunk consecutive tiles of
+// a grouped order (strips of kGroupM tile rows, walked column by column): about
+// kGroupM x (kXcdChunk / kGroupM) tiles whose A row panels and B column panels the XCD's L2
+// shares, while the 8 XCDs still sweep the grid together (same balance as the plain order).
+// The map is a bijection on [0, tiles): launch indices past the last whole round of
+// 8 * kXcdChunk keep their place.
+#ifndef MOAO_GEMM_REMAP
+#define MOAO_GEMM_REMAP 1
+#endif
+#ifndef MOAO_GEMM_XCD_CHUNK
+#define MOAO_GEMM_XCD_CHUNK 32
+#endif
+#ifndef MOAO_GEMM_GROUP_M
+#define MOAO_GEMM_GROUP_M 8
+#endif
+constexpr int kXcds = 8;
+constexpr int kXcdChunk = MOAO_GEMM_XCD_CHUNK;
+constexpr int kGroupM = MOAO_GEMM_GROUP_M;
+
+struct TileIdx {
+    int x, y, z;
+};
+
+__host__ __device__ inline TileIdx tile_order(long L, int gx, int gy, int gz) {
+    const long total = (long)gx * gy * gz;
+    if (kXcdChunk > 0) {
+        const long round = (long)kXcds * kXcdChunk;
+        if (L < total / round * round) {
+            const long c = L % kXcds, q = L / kXcds;
+            L = (q / kXcdChunk) * round + c * kXcdChunk + q % kXcdChunk;
+        }
+    }
+    const long slice = (long)gx * gy;
+    const int z = (int)(L / slice);
+    const long r = L % slice;
+    const long strip = (long)kGroupM * gy;
+    const int x0 = (int)(r / strip) * kGroupM;
+    const int gm = kGroupM < gx - x0 ? kGroupM : gx - x0;
+    const long rr = r - (long)x0 * gy;
+    return TileIdx{x0 + (int)(rr % gm), (int)(rr / gm), z};
+}
+
+__device__ __forceinline__ TileIdx block_tile() {
+    if (!MOAO_GEMM_REMAP) return TileIdx{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
+    const long L = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    return tile_order(L, gridDim.x, gridDim.y, gridDim.z);
+}
+
 template <typename T, int TB, int S, int TM>
 __global__ void __launch_bounds__(NT, TM == 128 ? 2 : 4) gemm_dl_kernel(GemmArgs<T> g) {
     using acc_t = typename Mfma<T>::acc_t;
@@ -276,11 +329,12 @@ __global__ void __launch_bounds__(NT, TM == 128 ? 2 : 4) gemm_dl_kernel(GemmArgs
     __shared__ T Bs[S][BSZ];
 
     if (g.skip_if && *(volatile const int*)g.skip_if != 0) return;
-    const GemmSeg<T> s = g.seg[blockIdx.z];
-    const int m0 = blockIdx.x * TM;
-    const int n0 = blockIdx.y * BN;
+    const TileIdx tl = block_tile();
+    const GemmSeg<T> s = g.seg[tl.z];
+    const int m0 = tl.x * TM;
+    const int n0 = tl.y * BN;
     if (m0 >= s.M) return;
-    const bool tri = g.tri && blockIdx.z == 0;
+    const bool tri = g.tri && tl.z == 0;
     if (tri && m0 + TM - 1 + g.tri_off < n0) return;
 
     const int tid = threadIdx.x;
