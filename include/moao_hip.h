@@ -255,6 +255,11 @@ int moao_hip_intersample_process(moao_isample* s, const double* Cvv, double* psf
 int moao_hip_intersample_process_dev(moao_isample* s, const double* dCvv, int batch,
                                      double* dPsf, void* stream);
 void moao_hip_intersample_destroy(moao_isample* s);
+/* Long-exposure PSF: dSum[i] += dPsf[i] for i < count (device memory, stream-ordered).  The
+ * sum the reference's TODO describes (chameleon/moao_chameleon.cpp:324-345: the cumulated PSF
+ * of each direction over the run), kept on the device so an observation loop that writes no
+ * short-exposure PSF never copies one to the host.                                          */
+int moao_hip_psf_accumulate_dev(const double* dPsf, long count, double* dSum, void* stream);
 
 /* The reference's own intersample boundary, same names, arguments and results
  * (common/tomo_struct/intersample.hpp:22-63, intersample.cpp:175-333):

@@ -403,8 +403,13 @@ int main(int argc, char** argv) {
         dCvv((size_t)std::max(nact, 1L) * std::max(nact, 1L) * nTarget),
         dDx((size_t)std::max(nact, 1L) * m), dTmp((size_t)std::max(nact, 1L) * m),
         dPsf((size_t)std::max(N, 1L) * std::max(N, 1L) * nTarget);
-    // long-exposure PSF of each target: the sum of its short-exposure PSFs
-    std::vector<double> cumul(isample ? (size_t)N * N * nTarget : 0, 0.0);
+    // long-exposure PSF of each target: the sum of its short-exposure PSFs, on the device
+    std::unique_ptr<Dev> dCumul;
+    if (isample) {
+        dCumul.reset(new Dev((size_t)N * N * nTarget));
+        std::vector<double> z((size_t)N * N * nTarget, 0.0);
+        check(moao_hip_memcpy_h2d(dCumul->p, z.data(), z.size() * 8), "zero");
+    }
     if (nact) check(moao_hip_memcpy_h2d(dDx.p, Dx.data(), Dx.size() * 8), "Dx upload");
     moao_tomo_view view;
 
@@ -472,11 +477,18 @@ int main(int argc, char** argv) {
                 check(moao_hip_intersample_process_dev(isample, dCvv.p, (int)nTarget, dPsf.p,
                                                        nullptr),
                       "intersample");
-                std::vector<double> psf((size_t)N * N * nTarget);
-                check(moao_hip_memcpy_d2h(psf.data(), dPsf.p, psf.size() * 8), "psf");
+                check(moao_hip_psf_accumulate_dev(dPsf.p, N * N * nTarget, dCumul->p, nullptr),
+                      "psf accumulate");
+                // short-exposure PSFs reach the host only when they are written
+                std::vector<double> psf;
+                if (!out.empty() && a.i("write_short")) {
+                    psf.resize((size_t)N * N * nTarget);
+                    check(moao_hip_memcpy_d2h(psf.data(), dPsf.p, psf.size() * 8), "psf");
+                } else {
+                    check(moao_hip_synchronize(), "sync");
+                }
                 t_psf.stop();
-                for (size_t e = 0; e < psf.size(); ++e) cumul[e] += psf[e];
-                if (!out.empty() && a.i("write_short"))
+                if (!psf.empty())
                     for (long g = 0; g < nTarget; ++g)
                         fits_write(out + "/psf_oloop" + std::to_string(r) + "_iloop" +
                                        std::to_string(o) + "_gal" + std::to_string(g) + ".fits",
@@ -486,6 +498,8 @@ int main(int argc, char** argv) {
     }
     t_glob.stop();
     if (isample && !out.empty() && maxrefine * maxobs > 0) {
+        std::vector<double> cumul((size_t)N * N * nTarget);
+        check(moao_hip_memcpy_d2h(cumul.data(), dCumul->p, cumul.size() * 8), "long exposure");
         const double w = 1.0 / (double)(maxrefine * maxobs);
         for (auto& v : cumul) v *= w;
         for (long g = 0; g < nTarget; ++g)

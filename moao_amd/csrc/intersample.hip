@@ -174,6 +174,14 @@ __global__ void final_kernel(const C* __restrict__ Y, double* __restrict__ psf, 
     if (2 * p + 1 < batch) psf[(size_t)(2 * p + 1) * N * N + (size_t)i * N + j] = (double)v.y;
 }
 
+// Long-exposure sum: sum[i] += psf[i] (one FP64 addition per element and step, so the sum
+// equals a host loop that adds the steps' PSFs in the same order).
+__global__ void psf_accumulate(const double* __restrict__ psf, long count,
+                               double* __restrict__ sum) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) sum[i] += psf[i];
+}
+
 template <typename R>
 int process(moao_isample* S, const double* dCvv, int batch, double* dPsf, hipStream_t s) {
     using C = typename Cplx<R>::type;
@@ -312,6 +320,20 @@ int moao_hip_intersample_create(long nidx, long np, long N, const long* idx, con
 }
 
 long moao_hip_intersample_n(const moao_isample* S) { return S ? S->N : MOAO_ERR_ARG; }
+
+int moao_hip_psf_accumulate_dev(const double* dPsf, long count, double* dSum, void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    if (count < 0 || (count > 0 && (!dPsf || !dSum))) {
+        set_error("psf_accumulate: invalid arguments");
+        return MOAO_ERR_ARG;
+    }
+    if (count == 0) return MOAO_OK;
+    hipStream_t s = moao::pick_stream(stream);
+    hipLaunchKernelGGL(moao::psf_accumulate, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                       s, dPsf, count, dSum);
+    MOAO_HIP_TRY(hipGetLastError());
+    return MOAO_OK;
+}
 
 int moao_hip_intersample_process_dev(moao_isample* S, const double* dCvv, int batch,
                                      double* dPsf, void* stream) {

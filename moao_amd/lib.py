@@ -208,6 +208,8 @@ _SIGS = {
                                                         ctypes.c_int, ctypes.c_void_p,
                                                         ctypes.c_void_p]),
     "moao_hip_intersample_destroy": (None, [ctypes.c_void_p]),
+    "moao_hip_psf_accumulate_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_long,
+                                                   ctypes.c_void_p, ctypes.c_void_p]),
     "moao_hip_intersample_prepare": (None, [ctypes.POINTER(IsampleStruct), ctypes.c_long,
                                             ctypes.c_long, ctypes.c_float, ctypes.c_char_p]),
     "moao_hip_intersample_init": (ctypes.c_int, [ctypes.POINTER(IsampleStruct)]),

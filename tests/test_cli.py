@@ -178,3 +178,13 @@ def test_long_exposure_psf(tmp_path):
     assert dims == (N, N)
     assert not np.array_equal(p0, p1)
     assert np.array_equal(lp_, (p0 + p1) * 0.5)
+    # without short-exposure files the PSFs stay on the device: the same long exposure
+    d2 = os.path.join(d, "noshort")
+    os.makedirs(d2)
+    r = subprocess.run([EXE, "--sys_path", d, "--atm_path", d, "--maxrefine", "1", "--maxobs",
+                        "2", "--psf", "1", "--out", d2, "--write_short", "0"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert not os.path.exists(os.path.join(d2, "psf_oloop0_iloop0_gal0.fits"))
+    lp2, _ = fits.read_image(os.path.join(d2, "psf_nbrefine1_maxobs2_gal0.fits"))
+    assert np.array_equal(lp2, lp_)

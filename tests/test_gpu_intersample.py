@@ -111,6 +111,22 @@ def test_paired_directions_uneven_weights(prec, tol, batch):
         assert err <= tol
 
 
+def test_psf_accumulate():
+    """moao_hip_psf_accumulate_dev: sum += psf, bit for bit the host's sum in the same order;
+    count 0 is a no-op and a negative count an error."""
+    rng = np.random.default_rng(5)
+    a, b = rng.random(70001), rng.random(70001)
+    dS, dA, dB = DeviceArray.from_host(np.zeros(70001)), DeviceArray.from_host(a), \
+        DeviceArray.from_host(b)
+    for d in (dA, dB):
+        check(lib().moao_hip_psf_accumulate_dev(ctypes.c_void_p(d.ptr), 70001,
+                                                ctypes.c_void_p(dS.ptr), None), "accumulate")
+    assert np.array_equal(dS.to_host(), (0.0 + a) + b)
+    assert lib().moao_hip_psf_accumulate_dev(None, 0, None, None) == 0
+    assert lib().moao_hip_psf_accumulate_dev(ctypes.c_void_p(dA.ptr), -1,
+                                             ctypes.c_void_p(dS.ptr), None) != 0
+
+
 def _anisotropic_inputs(N, nssp=10, seed=7):
     """Inputs that are not symmetric under a transpose: a stationary anisotropic Cvv, a
     separable anisotropic |FT(influence)|^2 and an elliptical OTF (as the oracle's pins)."""
