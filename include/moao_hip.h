@@ -236,6 +236,18 @@ int moao_hip_cee_cvv_dev(long nmeas, long nmeasts, long nact, const double* dCmm
                          double* dCpp, long ldCpp, double* dCpm, long ldCpm, const double* dR,
                          long ldR, const double* dDx, long ldDx, double* dCee, long ldCee,
                          double* dCvv, long ldCvv, double* dTmp, long ldTmp, void* stream);
+/* Cee/Cvv of ngal targets in one call: the observation step's per-target Cee_Cvv calls
+ * (lapack/moao_lapack.cpp:165-175) batched.  The targets share Cmm and Dx; target g's Cpp, Cpm,
+ * R, Cee, Cvv and Tmp blocks start g block strides in (g*ldCpp*nmeasts, g*ldCpm*nmeas,
+ * g*ldR*nmeas, g*ldCee*nmeasts, g*ldCvv*nact, g*ldTmp*nmeasts doubles), as the
+ * reconstructor's targets do.  Every output equals that of ngal moao_hip_cee_cvv_dev calls bit
+ * for bit; up to 8 targets' products share each launch.                                      */
+int moao_hip_cee_cvv_batched_dev(long nmeas, long nmeasts, long nact, long ngal,
+                                 const double* dCmm, long ldCmm, double* dCpp, long ldCpp,
+                                 double* dCpm, long ldCpm, const double* dR, long ldR,
+                                 const double* dDx, long ldDx, double* dCee, long ldCee,
+                                 double* dCvv, long ldCvv, double* dTmp, long ldTmp,
+                                 void* stream);
 
 /* ---------------------------------------------------------------- intersample PSF ---------
  * intersample_prepare/init/process/free (common/tomo_struct/intersample.hpp:46-63,

@@ -8,9 +8,10 @@
 // prof{snap*snapPerNight+obs}-atmos-night0.txt (obs = -1 for the refine step), Dx.fits (nact x
 // nmeasts) when present, idx.fits / abs2fi.fits / otftel.fits for the PSF.  Refine step:
 // matcov Cmm (type 1), Ctm of every target into R (type 3), reconstructor.  Observation step:
-// Cmm, then per target Ctm, Ctt (type 4), Cee/Cvv and the PSF (psf_oloop*_iloop*_gal*.fits in
-// --out).  Matrices stay in device memory between stages; --out also receives R.fits,
-// Cee.fits and Cvv.fits of the last step (FITS images of doubles, memory order, as writeFits).
+// Cmm, then per target Ctm, Ctt (type 4), Cee/Cvv (all targets in one batched call) and the
+// PSF (psf_oloop*_iloop*_gal*.fits in --out).  Matrices stay in device memory between stages;
+// --out also receives R.fits (every target) and target 0's Cee.fits and Cvv.fits of the last
+// step (FITS images of doubles, memory order, as writeFits).
 // --mode mixed (FP32 updates) or bf16 (split-BF16 updates) solves each target with the
 // mixed-precision reconstructor instead.  With --psf 1 the targets' PSFs of one observation
 // step are computed in one batched call, and the long-exposure PSF of each target (the mean of
@@ -398,10 +399,12 @@ int main(int argc, char** argv) {
     }
     t_pre.stop();
 
-    Dev dCmm((size_t)n * n), dR((size_t)m * n * nTarget), dCtm((size_t)m * n),
-        dCtt((size_t)m * m), dCee((size_t)m * m),
+    // every target's Ctm, Ctt, Cee, Cvv and Tmp side by side (block g at g block strides), so
+    // one batched Cee/Cvv call serves all targets of an observation step
+    Dev dCmm((size_t)n * n), dR((size_t)m * n * nTarget), dCtm((size_t)m * n * nTarget),
+        dCtt((size_t)m * m * nTarget), dCee((size_t)m * m * nTarget),
         dCvv((size_t)std::max(nact, 1L) * std::max(nact, 1L) * nTarget),
-        dDx((size_t)std::max(nact, 1L) * m), dTmp((size_t)std::max(nact, 1L) * m),
+        dDx((size_t)std::max(nact, 1L) * m), dTmp((size_t)std::max(nact, 1L) * m * nTarget),
         dPsf((size_t)std::max(N, 1L) * std::max(N, 1L) * nTarget);
     // long-exposure PSF of each target: the sum of its short-exposure PSFs, on the device
     std::unique_ptr<Dev> dCumul;
@@ -454,23 +457,28 @@ int main(int argc, char** argv) {
             check(moao_hip_matcov_tile_dev(dCmm.p, n, n, 0, 0, n, &view, 1, nullptr), "Cmm");
             check(moao_hip_synchronize(), "sync");
             t_matcov.stop();
+            t_matcov.start();
             for (long g = 0; g < nTarget; ++g) {
                 moao_tomo_set_target(tomo, (int)g);
                 moao_tomo_get_view(tomo, &view);
-                t_matcov.start();
-                check(moao_hip_matcov_tile_dev(dCtm.p, m, n, 0, 0, m, &view, 3, nullptr), "Ctm");
-                check(moao_hip_matcov_tile_dev(dCtt.p, m, m, 0, 0, m, &view, 4, nullptr), "Ctt");
-                check(moao_hip_synchronize(), "sync");
-                t_matcov.stop();
-                t_CeeCvv.start();
-                const long na = std::max(nact, 1L);
-                check(moao_hip_cee_cvv_dev(n, m, nact, dCmm.p, n, dCtt.p, m, dCtm.p, m,
-                                           dR.p + g * n * m, m, dDx.p, na, dCee.p, m,
-                                           dCvv.p + g * na * na, na, dTmp.p, na, nullptr),
-                      "Cee_Cvv");
-                check(moao_hip_synchronize(), "sync");
-                t_CeeCvv.stop();
+                check(moao_hip_matcov_tile_dev(dCtm.p + g * m * n, m, n, 0, 0, m, &view, 3,
+                                               nullptr),
+                      "Ctm");
+                check(moao_hip_matcov_tile_dev(dCtt.p + g * m * m, m, m, 0, 0, m, &view, 4,
+                                               nullptr),
+                      "Ctt");
             }
+            check(moao_hip_synchronize(), "sync");
+            t_matcov.stop();
+            // Cee/Cvv of every target (lapack/moao_lapack.cpp:165-175 calls it per target)
+            t_CeeCvv.start();
+            const long na = std::max(nact, 1L);
+            check(moao_hip_cee_cvv_batched_dev(n, m, nact, nTarget, dCmm.p, n, dCtt.p, m, dCtm.p,
+                                               m, dR.p, m, dDx.p, na, dCee.p, m, dCvv.p, na,
+                                               dTmp.p, na, nullptr),
+                  "Cee_Cvv");
+            check(moao_hip_synchronize(), "sync");
+            t_CeeCvv.stop();
             if (isample) {
                 // every target's PSF of this step in one batched call
                 t_psf.start();

@@ -12,6 +12,12 @@
 // so Cee = Ctt - Ctm R^T - R Ctm^T + R Cmm R^T and Cvv = Dx Cee Dx^T, with the reference's side
 // effects on Cpp (lower part = step 1 result) and Cpm (= R Cmm).  Unlike the reference, every
 // leading-dimension argument is honoured (the reference passes nmeasts/nmeas/nact regardless).
+//
+// The batched entry runs the chain for ngal targets (the observation step's per-target calls)
+// with each step's products of up to kMaxSeg targets in one launch (GEMM segments with their own
+// A, C and, for steps 1 and 6, B; Cmm and Dx shared): at m = 2,336 one target's products leave
+// most of a launch's last wave of tiles empty, eight targets' fill it.  Each element goes through
+// the same kernel arithmetic as in a call for its target alone, so the outputs are identical.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,17 +40,25 @@ __global__ void mirror_lower(const double* __restrict__ src, long lds, double* _
 
 }  // namespace
 
-int cee_cvv_dev(long n, long m, long nact, const double* Cmm, long ldCmm, double* Cpp,
-                long ldCpp, double* Cpm, long ldCpm, const double* R, long ldR,
-                const double* Dx, long ldDx, double* Cee, long ldCee, double* Cvv, long ldCvv,
-                double* Tmp, long ldTmp, hipStream_t s) {
-    if (n < 1 || m < 1 || nact < 0 || ldCmm < n || ldCpp < m || ldCpm < m || ldR < m ||
-        (nact > 0 && ldDx < nact) || ldCee < m || (nact > 0 && (ldCvv < nact || ldTmp < nact)) ||
-        n >= (1L << 31) || m >= (1L << 31) || nact >= (1L << 31)) {
+int cee_cvv_batched_dev(long n, long m, long nact, long ngal, const double* Cmm, long ldCmm,
+                        double* Cpp, long ldCpp, double* Cpm, long ldCpm, const double* R,
+                        long ldR, const double* Dx, long ldDx, double* Cee, long ldCee,
+                        double* Cvv, long ldCvv, double* Tmp, long ldTmp, hipStream_t s) {
+    if (n < 1 || m < 1 || nact < 0 || ngal < 1 || ldCmm < n || ldCpp < m || ldCpm < m ||
+        ldR < m || (nact > 0 && ldDx < nact) || ldCee < m ||
+        (nact > 0 && (ldCvv < nact || ldTmp < nact)) || n >= (1L << 31) || m >= (1L << 31) ||
+        nact >= (1L << 31)) {
         set_error("cee_cvv: invalid arguments");
         return MOAO_ERR_ARG;
     }
     const int N = (int)n, M = (int)m, A = (int)nact;
+    // target g's blocks: one block stride (leading dimension x columns) after target g-1's
+    auto Cpp_ = [&](long g) { return Cpp + g * ldCpp * m; };
+    auto Cpm_ = [&](long g) { return Cpm + g * ldCpm * n; };
+    auto R_ = [&](long g) { return R + g * ldR * n; };
+    auto Cee_ = [&](long g) { return Cee + g * ldCee * m; };
+    auto Cvv_ = [&](long g) { return Cvv + g * ldCvv * nact; };
+    auto Tmp_ = [&](long g) { return Tmp + g * ldTmp * m; };
     // Stream-ordered scratch for the symmetric operands' diagonal blocks (steps 2 and 7), so
     // calls on different streams never share it; without it those steps use the
     // register-staged kernel.
@@ -58,46 +72,112 @@ int cee_cvv_dev(long n, long m, long nact, const double* Cmm, long ldCmm, double
         sd.p = nullptr;
         (void)hipGetLastError();
     }
-    // 1: Cpp(lower) -= Cpm R^T + R Cpm^T
-    auto g = gemm_args<double>(M, M, N, -1.0, Cpm, ldCpm, R, ldR, 1, 1.0, Cpp, ldCpp);
-    g.tri = 1;
-    if (int rc = gemm_f64(g, s)) return rc;
-    g = gemm_args<double>(M, M, N, -1.0, R, ldR, Cpm, ldCpm, 1, 1.0, Cpp, ldCpp);
-    g.tri = 1;
-    if (int rc = gemm_f64(g, s)) return rc;
-    // 2: Cpm = R sym(Cmm)
-    g = gemm_args<double>(M, N, N, 1.0, R, ldR, Cmm, ldCmm, 2, 0.0, Cpm, ldCpm);
-    if (sd.p) {
+    // One launch per step for up to kMaxSeg targets: segment i is target g0 + i, with its own
+    // A, C and (steps 1 and 6) B.
+    using Seg = GemmSeg<double>;
+    auto batch = [&](GemmArgs<double> g, long g0, int ns, auto&& seg) {
+        g.nseg = ns;
+        for (int i = 0; i < ns; ++i) g.seg[i] = seg(g0 + i);
+        return gemm_f64(g, s);
+    };
+    for (long g0 = 0; g0 < ngal; g0 += kMaxSeg) {
+        const int ns = (int)std::min<long>(kMaxSeg, ngal - g0);
+        // 1: Cpp(lower) -= Cpm R^T + R Cpm^T
+        auto g = gemm_args<double>(M, M, N, -1.0, nullptr, ldCpm, R_(g0), ldR, 1, 1.0, nullptr,
+                                   ldCpp);
+        g.tri = 1;
+        g.tri_all = 1;
+        if (int rc = batch(g, g0, ns, [&](long t) {
+                Seg q{Cpm_(t), Cpp_(t), ldCpm, ldCpp, M};
+                q.B = R_(t);
+                return q;
+            }))
+            return rc;
+        g = gemm_args<double>(M, M, N, -1.0, nullptr, ldR, Cpm_(g0), ldCpm, 1, 1.0, nullptr,
+                              ldCpp);
+        g.tri = 1;
+        g.tri_all = 1;
+        if (int rc = batch(g, g0, ns, [&](long t) {
+                Seg q{R_(t), Cpp_(t), ldR, ldCpp, M};
+                q.B = Cpm_(t);
+                return q;
+            }))
+            return rc;
+    }
+    // 2: Cpm = R sym(Cmm), every target against the one Cmm
+    if (sd.p)
         if (int rc = gemm_symdiag(Cmm, ldCmm, N, sd.p, s)) return rc;
+    for (long g0 = 0; g0 < ngal; g0 += kMaxSeg) {
+        const int ns = (int)std::min<long>(kMaxSeg, ngal - g0);
+        auto g = gemm_args<double>(M, N, N, 1.0, nullptr, ldR, Cmm, ldCmm, 2, 0.0, nullptr, ldCpm);
         g.symdiag = sd.p;
+        if (int rc = batch(g, g0, ns, [&](long t) { return Seg{R_(t), Cpm_(t), ldR, ldCpm, M}; }))
+            return rc;
     }
-    if (int rc = gemm_f64(g, s)) return rc;
-    // 3-5: Cee = lower(Cpp) mirrored
-    {
-        ProfScope ps(kProfOther, 16.0 * m * m, s);
-        dim3 grid((M + 63) / 64, (M + 3) / 4);
-        hipLaunchKernelGGL(mirror_lower, grid, dim3(256), 0, s, Cpp, ldCpp, Cee, ldCee, M);
-        MOAO_HIP_TRY(hipGetLastError());
+    for (long g0 = 0; g0 < ngal; g0 += kMaxSeg) {
+        const int ns = (int)std::min<long>(kMaxSeg, ngal - g0);
+        // 3-5: Cee = lower(Cpp) mirrored
+        {
+            ProfScope ps(kProfOther, 16.0 * m * m * ns, s);
+            dim3 grid((M + 63) / 64, (M + 3) / 4);
+            for (int i = 0; i < ns; ++i)
+                hipLaunchKernelGGL(mirror_lower, grid, dim3(256), 0, s, Cpp_(g0 + i), ldCpp,
+                                   Cee_(g0 + i), ldCee, M);
+            MOAO_HIP_TRY(hipGetLastError());
+        }
+        // 6: Cee += Cpm R^T
+        auto g = gemm_args<double>(M, M, N, 1.0, nullptr, ldCpm, R_(g0), ldR, 1, 1.0, nullptr,
+                                   ldCee);
+        if (int rc = batch(g, g0, ns, [&](long t) {
+                Seg q{Cpm_(t), Cee_(t), ldCpm, ldCee, M};
+                q.B = R_(t);
+                return q;
+            }))
+            return rc;
     }
-    // 6: Cee += Cpm R^T
-    g = gemm_args<double>(M, M, N, 1.0, Cpm, ldCpm, R, ldR, 1, 1.0, Cee, ldCee);
-    if (int rc = gemm_f64(g, s)) return rc;
     if (A == 0) return MOAO_OK;
-    // 7: Tmp = Dx sym(lower(Cee))
-    g = gemm_args<double>(A, M, M, 1.0, Dx, ldDx, Cee, ldCee, 2, 0.0, Tmp, ldTmp);
-    if (sd.p) {
-        if (int rc = gemm_symdiag(Cee, ldCee, M, sd.p, s)) return rc;
-        g.symdiag = sd.p;
+    // 7: Tmp = Dx sym(lower(Cee)), one launch per target (each has its own symmetric operand)
+    for (long t = 0; t < ngal; ++t) {
+        auto g = gemm_args<double>(A, M, M, 1.0, Dx, ldDx, Cee_(t), ldCee, 2, 0.0, Tmp_(t), ldTmp);
+        if (sd.p) {
+            if (int rc = gemm_symdiag(Cee_(t), ldCee, M, sd.p, s)) return rc;
+            g.symdiag = sd.p;
+        }
+        if (int rc = gemm_f64(g, s)) return rc;
     }
-    if (int rc = gemm_f64(g, s)) return rc;
-    // 8: Cvv = Tmp Dx^T
-    g = gemm_args<double>(A, A, M, 1.0, Tmp, ldTmp, Dx, ldDx, 1, 0.0, Cvv, ldCvv);
-    return gemm_f64(g, s);
+    // 8: Cvv = Tmp Dx^T, every target against the one Dx
+    for (long g0 = 0; g0 < ngal; g0 += kMaxSeg) {
+        const int ns = (int)std::min<long>(kMaxSeg, ngal - g0);
+        auto g = gemm_args<double>(A, A, M, 1.0, nullptr, ldTmp, Dx, ldDx, 1, 0.0, nullptr, ldCvv);
+        if (int rc = batch(g, g0, ns, [&](long t) { return Seg{Tmp_(t), Cvv_(t), ldTmp, ldCvv, A}; }))
+            return rc;
+    }
+    return MOAO_OK;
+}
+
+int cee_cvv_dev(long n, long m, long nact, const double* Cmm, long ldCmm, double* Cpp,
+                long ldCpp, double* Cpm, long ldCpm, const double* R, long ldR,
+                const double* Dx, long ldDx, double* Cee, long ldCee, double* Cvv, long ldCvv,
+                double* Tmp, long ldTmp, hipStream_t s) {
+    return cee_cvv_batched_dev(n, m, nact, 1, Cmm, ldCmm, Cpp, ldCpp, Cpm, ldCpm, R, ldR, Dx,
+                               ldDx, Cee, ldCee, Cvv, ldCvv, Tmp, ldTmp, s);
 }
 
 }  // namespace moao
 
 extern "C" {
+
+int moao_hip_cee_cvv_batched_dev(long nmeas, long nmeasts, long nact, long ngal,
+                                 const double* dCmm, long ldCmm, double* dCpp, long ldCpp,
+                                 double* dCpm, long ldCpm, const double* dR, long ldR,
+                                 const double* dDx, long ldDx, double* dCee, long ldCee,
+                                 double* dCvv, long ldCvv, double* dTmp, long ldTmp,
+                                 void* stream) {
+    if (int rc = moao::ensure_init()) return rc;
+    return moao::cee_cvv_batched_dev(nmeas, nmeasts, nact, ngal, dCmm, ldCmm, dCpp, ldCpp, dCpm,
+                                     ldCpm, dR, ldR, dDx, ldDx, dCee, ldCee, dCvv, ldCvv, dTmp,
+                                     ldTmp, moao::pick_stream(stream));
+}
 
 int moao_hip_cee_cvv_dev(long nmeas, long nmeasts, long nact, const double* dCmm, long ldCmm,
                          double* dCpp, long ldCpp, double* dCpm, long ldCpm, const double* dR,

@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(NT, WMB == 4 ? 2 : 4) gemm_kernel(GemmArgs<T> 
     const int m0 = blockIdx.x * TM;
     const int n0 = blockIdx.y * BN;
     if (m0 >= s.M) return;
-    const bool tri = g.tri && blockIdx.z == 0;
+    const bool tri = g.tri && (blockIdx.z == 0 || g.tri_all);
     if (tri && m0 + TM - 1 + g.tri_off < n0) return;
 
     const int tid = threadIdx.x;
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(NT, WMB == 4 ? 2 : 4) gemm_kernel(GemmArgs<T> 
     const unsigned aoff = (unsigned)min(arow, Mc) * (unsigned)sizeof(T);  // bytes
     const bool arow_ok = arow < M;
     const T* __restrict__ A = s.A;
-    const T* __restrict__ Bm = g.B;
+    const T* __restrict__ Bm = s.B ? s.B : g.B;
     // TB=0 / crossing-slice pattern: per-lane column offsets, computed once per tile
     long bcol[RV];
 #pragma unroll
@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(NT, TM == 128 ? 2 : 4) gemm_dl_kernel(GemmArgs
     const int m0 = tl.x * TM;
     const int n0 = tl.y * BN;
     if (m0 >= s.M) return;
-    const bool tri = g.tri && tl.z == 0;
+    const bool tri = g.tri && (tl.z == 0 || g.tri_all);
     if (tri && m0 + TM - 1 + g.tri_off < n0) return;
 
     const int tid = threadIdx.x;
@@ -355,7 +355,8 @@ __global__ void __launch_bounds__(NT, TM == 128 ? 2 : 4) gemm_dl_kernel(GemmArgs
     // symmetric B below the diagonal block: rows k of columns n0..); symmetric slices that
     // cross the diagonal block read its full symmetric copy in g.symdiag.
     const bool isA = wave < 2;
-    const T* base = isA ? s.A + m0 : (TB == 0 ? g.B + (long)n0 * g.ldb : g.B + n0);
+    const T* Bm = s.B ? s.B : g.B;
+    const T* base = isA ? s.A + m0 : (TB == 0 ? Bm + (long)n0 * g.ldb : Bm + n0);
     const long ld = isA ? s.lda : g.ldb;
     const T* diagblk = TB == 2 ? g.symdiag + (long)(n0 / BN) * BN * BN : nullptr;
     const int nv = isA ? min(M - m0, TM) : min(N - n0, BN);  // valid rows (cols for K x N)
@@ -392,7 +393,7 @@ __global__ void __launch_bounds__(NT, TM == 128 ? 2 : 4) gemm_dl_kernel(GemmArgs
         } else {
             // slab rows k0..k0+BK-1 of columns n0..n0+nv-1; the range ends at the last valid
             // element so nothing past the matrix is read
-            const T* slab = TB == 0 ? base : g.B + (long)n0 * ld;
+            const T* slab = TB == 0 ? base : Bm + (long)n0 * ld;
             const int kv = min(BK, K - k0);
             const int nrec = ((nv - 1) * (int)ld + kv) * (int)sizeof(T);
             const __amdgpu_buffer_rsrc_t r =
@@ -512,7 +513,7 @@ __global__ void __launch_bounds__(NT, TM == 128 ? 1 : 2) gemm_split2_kernel(Gemm
     const int m0 = blockIdx.x * TM;
     const int n0 = blockIdx.y * BN;
     if (m0 >= s.M) return;
-    const bool tri = g.tri && blockIdx.z == 0;
+    const bool tri = g.tri && (blockIdx.z == 0 || g.tri_all);
     if (tri && m0 + TM - 1 + g.tri_off < n0) return;
 
     const int tid = threadIdx.x;
@@ -679,7 +680,13 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
         return MOAO_ERR_ARG;
     }
     int maxM = 0;
-    for (int i = 0; i < g.nseg; ++i) maxM = g.seg[i].M > maxM ? g.seg[i].M : maxM;
+    for (int i = 0; i < g.nseg; ++i) {
+        maxM = g.seg[i].M > maxM ? g.seg[i].M : maxM;
+        if (g.seg[i].B && g.transb == 2) {
+            set_error("gemm: a symmetric B is shared by all segments");
+            return MOAO_ERR_ARG;
+        }
+    }
     if (maxM == 0 || g.N == 0) return MOAO_OK;
     dim3 grid((maxM + BM - 1) / BM, (g.N + BN - 1) / BN, g.nseg);
     if (grid.y > 65535) {
@@ -690,7 +697,7 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
     if (profiling()) {
         for (int i = 0; i < g.nseg; ++i) {
             double area = (double)g.seg[i].M * g.N;
-            if (i == 0 && g.tri) {  // elements with row + tri_off >= col
+            if ((i == 0 || g.tri_all) && g.tri) {  // elements with row + tri_off >= col
                 area = 0;
                 for (int c = 0; c < g.N; ++c)
                     area += std::max(0, g.seg[i].M - std::max(0, c - g.tri_off));
@@ -707,6 +714,11 @@ int launch(const GemmArgs<T>& g, hipStream_t stream) {
                 set_error("gemm: the planes product takes op(B) = B or B^T only");
                 return MOAO_ERR_ARG;
             }
+            for (int i = 0; i < g.nseg; ++i)
+                if (g.seg[i].B) {
+                    set_error("gemm: the planes product takes one B for all segments");
+                    return MOAO_ERR_ARG;
+                }
             if (small) {
                 grid.x = (maxM + 63) / 64;
                 hipLaunchKernelGGL((gemm_split2_kernel<64>), grid, dim3(NT), 0, stream, g);

@@ -7,9 +7,10 @@
 // only the lower triangle is read (transb = 2, the B operand of dsymm(Right, Lower)).
 // Up to kMaxSeg row segments share one B and one
 // launch (grid.z = segment): the reconstructor uses this to update the Cmm rows and the Ctm
-// rows of every target with a single launch.  Segment 0 may be restricted to its lower
-// trapezoid (row + tri_off >= col): tiles wholly above that line are skipped and elements
-// above it are not written.
+// rows of every target with a single launch.  A segment may bring its own B of the same shape
+// (GemmSeg::B: a batch of independent products in one launch, e.g. one per target).  Segment 0
+// (every segment with tri_all) may be restricted to its lower trapezoid (row + tri_off >= col):
+// tiles wholly above that line are skipped and elements above it are not written.
 //
 // In-place use: when C_g == A_g and N <= the kernel's N tile (one column tile), each workgroup
 // reads its whole row block of A before it writes C, and no other workgroup touches those rows,
@@ -20,7 +21,7 @@
 
 namespace moao {
 
-constexpr int kMaxSeg = 4;
+constexpr int kMaxSeg = 8;
 
 // Split-BF16 planes of an FP32 matrix region: the hi term of element (row, k) at
 // hi[row * ld + k], the lo term at hi[ps + row * ld + k] (k contiguous within a row; ld a
@@ -38,6 +39,7 @@ struct GemmSeg {
     long lda, ldc;
     int M;
     Planes ap;  // split = 2: A's planes (row = row of this segment)
+    const T* B;  // optional: this segment's own B (same shape and ldb), else GemmArgs::B
 };
 
 template <typename T>
@@ -50,6 +52,7 @@ struct GemmArgs {
     int transb;
     int tri;      // segment 0 lower-trapezoid restriction
     int tri_off;
+    int tri_all;  // with tri: every segment restricted, not only segment 0
     T alpha, beta;
     // transb = 2 only, optional: full symmetric copies of B's 128 x 128 diagonal blocks
     // (gemm_symdiag); with it the launch runs on the direct-to-LDS kernel.
@@ -99,6 +102,7 @@ inline GemmArgs<T> gemm_args(int M, int N, int K, T alpha, const T* A, long lda,
     g.transb = transb;
     g.tri = 0;
     g.tri_off = 0;
+    g.tri_all = 0;
     g.alpha = alpha;
     g.beta = beta;
     g.symdiag = nullptr;

@@ -197,6 +197,8 @@ _SIGS = {
                          + [_c_double_p, ctypes.c_long] * 8),
     "moao_hip_cee_cvv_dev": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_long]
                              + [ctypes.c_void_p, ctypes.c_long] * 8 + [ctypes.c_void_p]),
+    "moao_hip_cee_cvv_batched_dev": (ctypes.c_int, [ctypes.c_long] * 4
+                                     + [ctypes.c_void_p, ctypes.c_long] * 8 + [ctypes.c_void_p]),
     "moao_intersample_default_n": (ctypes.c_long, [ctypes.c_long]),
     "moao_hip_intersample_create": (ctypes.c_int, [ctypes.c_long, ctypes.c_long, ctypes.c_long,
                                                    _c_long_p, _c_double_p, _c_double_p,

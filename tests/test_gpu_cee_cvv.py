@@ -103,3 +103,53 @@ def test_cee_cvv_large_synthetic():
     print(f"    Cee {e_cee:.2e} Cvv {e_cvv:.2e} Cpm {e_cpm:.2e}", flush=True)
     assert e_cee <= 1e-11 and e_cvv <= 1e-11 and e_cpm <= 1e-12
     assert np.all(np.isfinite(Cee)) and np.all(np.isfinite(Cvv))
+
+
+@pytest.mark.parametrize("ngal", [3, 9])
+def test_batched_targets_equal_per_target_calls(ngal):
+    """moao_hip_cee_cvv_batched_dev (every target's products in shared launches, 8 targets per
+    launch) against one moao_hip_cee_cvv_dev call per target: every output and side effect
+    identical bit for bit, ragged sizes, 9 targets = two launches per step."""
+    import ctypes
+    from moao_amd.device import DeviceArray
+    from moao_amd.lib import check, lib
+    print(f"[cee] batched ngal={ngal}", flush=True)
+    n, m, nact = 700, 301, 137
+    rng = np.random.default_rng(ngal)
+    G = rng.standard_normal((n, n))
+    Cmm = np.asfortranarray(G @ G.T / n + np.eye(n))
+    Cmm[np.triu_indices(n, 1)] = 1e300  # only the lower triangle may be read
+    Dx = np.asfortranarray(rng.standard_normal((nact, m)))
+    Ctt = [rng.standard_normal((m, m)) for _ in range(ngal)]
+    Ctt = [np.asfortranarray(a + a.T) for a in Ctt]
+    Ctm = [np.asfortranarray(rng.standard_normal((m, n))) for _ in range(ngal)]
+    R = [np.asfortranarray(rng.standard_normal((m, n))) for _ in range(ngal)]
+    vp = ctypes.c_void_p
+    dCmm, dDx = DeviceArray.from_host(Cmm), DeviceArray.from_host(Dx)
+    # batched: target g's blocks side by side along the columns
+    b = {"Cpp": DeviceArray.from_host(np.hstack(Ctt)), "Cpm": DeviceArray.from_host(np.hstack(Ctm)),
+         "R": DeviceArray.from_host(np.hstack(R)), "Cee": DeviceArray((m, m * ngal)),
+         "Cvv": DeviceArray((nact, nact * ngal)), "Tmp": DeviceArray((nact, m * ngal))}
+    check(lib().moao_hip_cee_cvv_batched_dev(
+        n, m, nact, ngal, vp(dCmm.ptr), n, vp(b["Cpp"].ptr), m, vp(b["Cpm"].ptr), m,
+        vp(b["R"].ptr), m, vp(dDx.ptr), nact, vp(b["Cee"].ptr), m, vp(b["Cvv"].ptr), nact,
+        vp(b["Tmp"].ptr), nact, None), "cee_cvv_batched")
+    got = {k: v.to_host() for k, v in b.items()}
+    width = {"Cpp": m, "Cpm": n, "R": n, "Cee": m, "Cvv": nact, "Tmp": m}
+    for g in range(ngal):
+        s = {"Cpp": DeviceArray.from_host(Ctt[g]), "Cpm": DeviceArray.from_host(Ctm[g]),
+             "R": DeviceArray.from_host(R[g]), "Cee": DeviceArray((m, m)),
+             "Cvv": DeviceArray((nact, nact)), "Tmp": DeviceArray((nact, m))}
+        check(lib().moao_hip_cee_cvv_dev(
+            n, m, nact, vp(dCmm.ptr), n, vp(s["Cpp"].ptr), m, vp(s["Cpm"].ptr), m,
+            vp(s["R"].ptr), m, vp(dDx.ptr), nact, vp(s["Cee"].ptr), m, vp(s["Cvv"].ptr), nact,
+            vp(s["Tmp"].ptr), nact, None), "cee_cvv")
+        for k, v in s.items():
+            w = width[k]
+            assert np.array_equal(got[k][:, g * w:(g + 1) * w], v.to_host()), (g, k)
+    # and one target against the closed form (the per-target path's own parity is above)
+    Cl = np.tril(Cmm) + np.tril(Cmm, -1).T
+    g = ngal - 1
+    Cee_cf = Ctt[g] - Ctm[g] @ R[g].T - R[g] @ Ctm[g].T + R[g] @ Cl @ R[g].T
+    Cee_g = got["Cee"][:, g * m:(g + 1) * m]
+    assert rel_fro(Cee_g, Cee_cf) <= 1e-12
