@@ -49,12 +49,26 @@ struct moao_isample {
 namespace moao {
 namespace {
 
+// Workgroup order.  Entry k of bin b + 1 is usually the Cvv element next to entry k of bin b
+// (the same actuator pair shifted by one actuator), so consecutive bins read the same cache
+// lines.  The dispatcher deals workgroups to the 8 XCDs round robin; launch index L is
+// re-mapped so that each XCD's runs of kGatherRun workgroups take 4 * kGatherRun consecutive
+// bins, which then meet in that XCD's L2, while the XCDs still share every part of the map
+// (a bijection on [0, workgroups): indices past the last whole round keep their place).
+constexpr int kGatherRun = 16;
+
 __global__ void map_gather(const double* __restrict__ cvv, long nidx, const int* __restrict__ off,
                            const int* __restrict__ ent, const double* __restrict__ div,
                            double* __restrict__ Map, long nmap) {
-    const long bin = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    long L = blockIdx.x + (long)gridDim.x * blockIdx.y;
+    const long total = (long)gridDim.x * gridDim.y, round = 8L * kGatherRun;
+    if (L < total / round * round) {
+        const long c = L % 8, q = L / 8;
+        L = (q / kGatherRun) * round + c * kGatherRun + q % kGatherRun;
+    }
+    const long bin = (L % gridDim.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int b = blockIdx.y;
+    const int b = (int)(L / gridDim.x);
     if (bin >= nmap) return;
     const double* c = cvv + (size_t)b * nidx;
     double acc = 0.0;
