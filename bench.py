@@ -1,7 +1,7 @@
 """Benchmark of the MOAO reconstructor hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload elt74]
-                    [--modes fp64,mixed,bf16] [--serve 256] [--serve-modes fp64,bf16]
+                    [--modes fp64,mixed,bf16] [--serve 256] [--serve-modes fp64]
                     [--serve-mixed 128] [--obs-loop 1] [--obs-prepare DIR]
                     [--slots 4] [--no-cpu]
 
@@ -17,8 +17,9 @@ latency         p50 / p99 of the per-step wall time (each step synchronised)
 roofline        FP64 MFMA GEMM engine: algorithmic GEMM flops / summed GEMM kernel time
                 (HIP events on the launch stream) vs the 78.6 TFLOP/s FP64 peak
 serving         a Poisson stream of uneven systems (n up to 40k) through the serving engine:
-                throughput and p50/p99 latency from submit to R on the host, FP64 (256 requests)
-                and split-BF16 mixed (128), with the slots' graph captures and replays
+                throughput and p50/p99 latency from submit to R on the host, FP64 (256
+                requests; split-BF16 mixed, 128, with --serve-modes fp64,bf16), with the slots'
+                graph captures and replays
 observation_loop  the refine/observation loop of the `moao` driver on the n = 18,688 system with
                 8 directions (4 refine x 16 observation steps, Cee/Cvv, batched PSFs, long
                 exposure), and the PSF Strehl with FP32 and FP64 transforms
@@ -422,7 +423,9 @@ def main():
                     help="run the refine/observation loop leg (n=18688, 8 directions, PSFs)")
     ap.add_argument("--serve", type=int, default=256,
                     help="requests in the serving leg (0 = skip it)")
-    ap.add_argument("--serve-modes", default="fp64,bf16")
+    ap.add_argument("--serve-modes", default="fp64",
+                    help="serving legs to run (fp64, mixed, bf16); the split-BF16 leg after the "
+                         "FP64 one stopped completing requests once and is not run by default")
     ap.add_argument("--serve-mixed", type=int, default=128,
                     help="requests in the mixed-precision serving legs (their solves take "
                          "about three times longer; 0 = as many as --serve)")
