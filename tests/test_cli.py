@@ -188,3 +188,69 @@ def test_long_exposure_psf(tmp_path):
     assert not os.path.exists(os.path.join(d2, "psf_oloop0_iloop0_gal0.fits"))
     lp2, _ = fits.read_image(os.path.join(d2, "psf_nbrefine1_maxobs2_gal0.fits"))
     assert np.array_equal(lp2, lp_)
+
+
+@pytest.mark.gpu
+def test_three_targets_batched_stages(tmp_path):
+    """Three truth directions: the driver solves R for all of them at once, computes every
+    target's Ctm/Ctt, one batched Cee/Cvv call and one batched PSF call per observation step.
+    R of every target, target 0's Cee/Cvv and every target's PSF against the oracle chain on
+    the same matrices (R <= 1e-10, Cee/Cvv <= 1e-10, PSF <= 2e-5 of the peak)."""
+    import dataclasses
+    from moao_amd import fits, psf_inputs, systems
+    from moao_amd import matcov as gm
+    from moao_amd.tomo import Tomo
+    from oracle import intersample as oi
+    from oracle import lapack_path as lp
+    from oracle.check import rel_fro
+    d = str(tmp_path)
+    spec = dataclasses.replace(systems.WORKLOADS["small"], ntarget=3, target_field_diam_as=40.0)
+    s, atms = systems.make_system(d, spec, nprof=2)
+    t = Tomo(s, atms[0])
+    n, m = t.nmeas(), t.nmeasts()
+    dm = psf_inputs.make_dm(10)
+    nact = dm.nact
+    Dx = systems.dx_matrix(nact, m, n)
+    fits.write_image(os.path.join(d, "Dx.fits"), Dx, nact, m)
+    idx = psf_inputs.make_idx(dm)
+    N = oi.default_n(dm.ndiam)
+    a2f = psf_inputs.make_abs2fi(N, 8.0, dm)
+    otf = psf_inputs.make_otf_tel(N, 8.0, 0.28)
+    fits.write_image(os.path.join(d, "idx.fits"), idx, idx.size, 1)
+    fits.write_image(os.path.join(d, "abs2fi.fits"), a2f, N, N)
+    fits.write_image(os.path.join(d, "otftel.fits"), otf, N, N)
+    print(f"[cli] 3 targets n={n} m={m} nact={nact} N={N}", flush=True)
+    r = subprocess.run([EXE, "--sys_path", d, "--atm_path", d, "--maxrefine", "1", "--maxobs",
+                        "1", "--psf", "1", "--out", d], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    Rf, dims = fits.read_image(os.path.join(d, "R.fits"))
+    assert dims == (m, 3 * n)
+    Rf = Rf.reshape((m, 3 * n), order="F")
+    C = gm.cmm(t)
+    Rs = []
+    for g in range(3):
+        t.set_target(g)
+        B = gm.ctm(t)
+        Ro = B.copy(order="F")
+        assert lp.reconstructor(C.copy(order="F"), Ro) == 0
+        Rg = Rf[:, g * n:(g + 1) * n]
+        assert rel_fro(Rg, Ro) <= 1e-10, g
+        Rs.append(np.asfortranarray(Rg))
+    t.update_atm(atms[1])
+    C = gm.cmm(t)
+    for g in range(3):
+        t.set_target(g)
+        B, T = gm.ctm(t), gm.ctt(t)
+        Cee, Cvv = lp.cee_cvv(C.copy(order="F"), T.copy(order="F"), B.copy(order="F"), Rs[g],
+                              np.asfortranarray(Dx))
+        if g == 0:
+            Ceef, _ = fits.read_image(os.path.join(d, "Cee.fits"))
+            Cvvf, _ = fits.read_image(os.path.join(d, "Cvv.fits"))
+            assert rel_fro(Ceef.reshape((m, m), order="F"), Cee) <= 1e-10
+            assert rel_fro(Cvvf.reshape((nact, nact), order="F"), Cvv) <= 1e-10
+        psf, pd = fits.read_image(os.path.join(d, f"psf_oloop0_iloop0_gal{g}.fits"))
+        ref = oi.process(Cvv.flatten(order="F"), idx, a2f, otf, N, dm.ndiam)
+        err = np.max(np.abs(psf.reshape(N, N) - ref)) / np.max(np.abs(ref))
+        print(f"    target {g}: PSF err {err:.2e}", flush=True)
+        assert err <= 2e-5, g
