@@ -1,7 +1,9 @@
 // Cee / Cvv error covariances (MOAO_LAPACK::Cee_Cvv, lapack/compute/cee_cvv.cpp:50-124) on gfx950.
 //
 // The reference chain and what runs here, all FP64 on the MFMA GEMM engine:
-//   1 dsyr2k(L,N, -1, Cpm, R, 1, Cpp)       two lower-trapezoid GEMM launches (A B^T, B A^T)
+//   1 dsyr2k(L,N, -1, Cpm, R, 1, Cpp)       X = Cpm R^T (one GEMM) and Cpp -= X, Cpp -= X^T on
+//                                           the lower triangle (one elementwise kernel); two
+//                                           lower-trapezoid GEMMs when X has no scratch
 //   2 dsymm(R,L, 1, Cmm, R, 0, Cpm)         GEMM with B = Cmm read from its lower triangle
 //   3-5 dlaset(U) + dlacpy(L) + in-place geadd('T') + dscal(0.5) on the diagonal
 //                                           one kernel: Cee = lower(Cpp) mirrored (the net
@@ -38,6 +40,19 @@ __global__ void mirror_lower(const double* __restrict__ src, long lds, double* _
     dst[i + (long)j * ldd] = i >= j ? src[i + (long)j * lds] : src[j + (long)i * lds];
 }
 
+// Cpp[i, j] = (Cpp[i, j] - X[i, j]) - X[j, i] for i >= j, target blockIdx.z: the lower part of
+// Cpp - X - X^T, rounded in the order of two accumulating products Cpp -= X, Cpp -= X^T.
+__global__ void syr2_apply(const double* __restrict__ X, long xstride, double* Cpp, long ldCpp,
+                           long cstride, int m) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= m || j >= m || i < j) return;
+    const double* x = X + blockIdx.z * xstride;
+    double* c = Cpp + blockIdx.z * cstride + i + (long)j * ldCpp;
+    const double v = *c - x[i + (long)j * m];
+    *c = v - x[j + (long)i * m];
+}
+
 }  // namespace
 
 int cee_cvv_batched_dev(long n, long m, long nact, long ngal, const double* Cmm, long ldCmm,
@@ -66,10 +81,18 @@ int cee_cvv_batched_dev(long n, long m, long nact, long ngal, const double* Cmm,
         double* p = nullptr;
         hipStream_t s;
         ~Scratch() { if (p) (void)hipFreeAsync(p, s); }
-    } sd{nullptr, s};
+    } sd{nullptr, s}, sx{nullptr, s};
     if (hipMallocAsync(reinterpret_cast<void**>(&sd.p),
                        sizeof(double) * gemm_symdiag_elems(std::max(n, m)), s) != hipSuccess) {
         sd.p = nullptr;
+        (void)hipGetLastError();
+    }
+    // step 1's product X = Cpm R^T for up to kMaxSeg targets (m x m each); without it step 1
+    // runs as two lower-trapezoid products
+    const long nx = std::min<long>(ngal, kMaxSeg);
+    if (hipMallocAsync(reinterpret_cast<void**>(&sx.p), sizeof(double) * m * m * nx, s) !=
+        hipSuccess) {
+        sx.p = nullptr;
         (void)hipGetLastError();
     }
     // One launch per step for up to kMaxSeg targets: segment i is target g0 + i, with its own
@@ -82,7 +105,26 @@ int cee_cvv_batched_dev(long n, long m, long nact, long ngal, const double* Cmm,
     };
     for (long g0 = 0; g0 < ngal; g0 += kMaxSeg) {
         const int ns = (int)std::min<long>(kMaxSeg, ngal - g0);
-        // 1: Cpp(lower) -= Cpm R^T + R Cpm^T
+        // 1: Cpp(lower) -= Cpm R^T + R Cpm^T.  (R Cpm^T)[i, j] = X[j, i] bit for bit with
+        // X = Cpm R^T (the same products, accumulated in the same k order), so one full
+        // product and an elementwise pass give exactly the two lower-trapezoid products'
+        // result, with all of one launch's tiles busy instead of half of two launches'.
+        if (sx.p) {
+            auto g = gemm_args<double>(M, M, N, 1.0, nullptr, ldCpm, R_(g0), ldR, 1, 0.0, nullptr,
+                                       m);
+            if (int rc = batch(g, g0, ns, [&](long t) {
+                    Seg q{Cpm_(t), sx.p + (t - g0) * m * m, ldCpm, m, M};
+                    q.B = R_(t);
+                    return q;
+                }))
+                return rc;
+            ProfScope ps(kProfOther, 24.0 * m * m / 2 * ns, s);
+            dim3 grid((M + 63) / 64, (M + 3) / 4, ns);
+            hipLaunchKernelGGL(syr2_apply, grid, dim3(256), 0, s, sx.p, m * m, Cpp_(g0),
+                               ldCpp, ldCpp * m, M);
+            MOAO_HIP_TRY(hipGetLastError());
+            continue;
+        }
         auto g = gemm_args<double>(M, M, N, -1.0, nullptr, ldCpm, R_(g0), ldR, 1, 1.0, nullptr,
                                    ldCpp);
         g.tri = 1;

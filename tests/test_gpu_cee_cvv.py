@@ -147,6 +147,18 @@ def test_batched_targets_equal_per_target_calls(ngal):
         for k, v in s.items():
             w = width[k]
             assert np.array_equal(got[k][:, g * w:(g + 1) * w], v.to_host()), (g, k)
+    # step 1 (dsyr2k) is one product X = Cpm R^T and Cpp -= X, Cpp -= X^T: its lower part
+    # equals the two accumulating products Cpp -= Cpm R^T, Cpp -= R Cpm^T bit for bit
+    from moao_amd.reconstructor import dgemm_dev
+    for g in (0, ngal - 1):
+        dC, dP, dRg = DeviceArray.from_host(Ctt[g]), DeviceArray.from_host(Ctm[g]), \
+            DeviceArray.from_host(R[g])
+        dgemm_dev(1, m, m, n, -1.0, dP.ptr, m, dRg.ptr, m, 1.0, dC.ptr, m)
+        dgemm_dev(1, m, m, n, -1.0, dRg.ptr, m, dP.ptr, m, 1.0, dC.ptr, m)
+        low = np.tril_indices(m)
+        assert np.array_equal(got["Cpp"][:, g * m:(g + 1) * m][low], dC.to_host()[low]), g
+        up = np.triu_indices(m, 1)
+        assert np.array_equal(got["Cpp"][:, g * m:(g + 1) * m][up], Ctt[g][up]), g
     # and one target against the closed form (the per-target path's own parity is above)
     Cl = np.tril(Cmm) + np.tril(Cmm, -1).T
     g = ngal - 1
