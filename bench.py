@@ -239,9 +239,9 @@ def stages(tomo, n, m, dCmm0, dCtm0, dR, ndir=8, N=2048, dm_nssp=79, reps=2):
     f = prof.result["fft"]
     # algorithmic HBM bytes of one direction: Cvv gather (8 B per entry + 4 B index) plus
     # half of 3 complex 2-D FFTs of N x N FP32 (two directions share each transform; each
-    # 2 row passes + 2 transposes, read + write 8 B), half of the pointwise passes
+    # 2 row passes + 1 transpose, read + write 8 B), half of the pointwise passes
     # (~6 N^2 x 8 B per pair) and the FP64 PSF write
-    byt = ndir * (12.0 * nact * nact + 1.5 * 4 * 2 * 8.0 * N * N + 3 * 8.0 * N * N
+    byt = ndir * (12.0 * nact * nact + 1.5 * 3 * 2 * 8.0 * N * N + 3 * 8.0 * N * N
                   + 8.0 * N * N)
     psf = dP.to_host()[:N * N].reshape(N, N)
     out["psf"] = {"N": N, "directions": ndir, "nact": nact, "precision": "fp32 FFT",

@@ -152,9 +152,27 @@ int fft2d(typename Cplx<R>::type* X, typename Cplx<R>::type* Y, int N, int batch
     return MOAO_OK;
 }
 
+template <typename R>
+int fft2d_t(typename Cplx<R>::type* X, typename Cplx<R>::type* Y, int N, int batch, int dir,
+            const typename Cplx<R>::type* tw, hipStream_t s) {
+    using C = typename Cplx<R>::type;
+    if (N < 32 || N > 4096 || (N & (N - 1)) || batch < 1) {
+        set_error("fft2d: N must be a power of two in [32, 4096]");
+        return MOAO_ERR_ARG;
+    }
+    ProfScope ps(kProfFft, 3.0 * 2.0 * sizeof(C) * (double)N * N * batch, s);
+    const dim3 tg(N / 32, N / 32, batch);
+    if (int rc = rows<R>(X, N, N * batch, dir, tw, s)) return rc;
+    hipLaunchKernelGGL(transpose<C>, tg, dim3(256), 0, s, X, Y, N);
+    MOAO_HIP_TRY(hipGetLastError());
+    return rows<R>(Y, N, N * batch, dir, tw, s);
+}
+
 template int fft_twiddles<float>(int, float2**, hipStream_t);
 template int fft_twiddles<double>(int, double2**, hipStream_t);
 template int fft2d<float>(float2*, float2*, int, int, int, const float2*, hipStream_t);
 template int fft2d<double>(double2*, double2*, int, int, int, const double2*, hipStream_t);
+template int fft2d_t<float>(float2*, float2*, int, int, int, const float2*, hipStream_t);
+template int fft2d_t<double>(double2*, double2*, int, int, int, const double2*, hipStream_t);
 
 }  // namespace moao

@@ -22,4 +22,11 @@ template <typename R>
 int fft2d(typename Cplx<R>::type* X, typename Cplx<R>::type* scratch, int N, int batch, int dir,
           const typename Cplx<R>::type* tw, hipStream_t s);
 
+// The same transform without its final transpose: the result of X goes to Y transposed
+// (Y[j][i] = F[i][j]); X is overwritten.  Since the 2-D transform commutes with transposition,
+// a transposed input gives the plain-layout result.
+template <typename R>
+int fft2d_t(typename Cplx<R>::type* X, typename Cplx<R>::type* Y, int N, int batch, int dir,
+            const typename Cplx<R>::type* tw, hipStream_t s);
+
 }  // namespace moao

@@ -11,7 +11,9 @@
 //                                                 transform costs half a complex one); the two
 //                                                 spectra separated by Hermitian symmetry, the
 //                                                 half-spectrum product extended to the full
-//                                                 grid by the mirror the c2r transform assumes
+//                                                 grid by the mirror the c2r transform assumes;
+//                                                 transforms skip their final transpose, the
+//                                                 spectra are held transposed
 //   dphi = exp(-0.5 (norm - dphi/fact2*fact))     dphi_kernel (fftshift folded in)
 //   _eclat; tmp = otf_tel[j*N+i]*dphi[j*N+i]; c2r spec2 (mirror as above), backward FFT
 //   _eclat -> PSF                                 final_kernel (fftshift folded in)
@@ -104,30 +106,33 @@ __global__ void embed(const double* __restrict__ Map, long nmap, C* X, int N, in
     X[(size_t)p * N * N + (size_t)i * N + j] = C{re, im};
 }
 
-// Y[i][j] = (Ga, Gb) with G(i, j) = Re F(i', j') * w[j'*N + i'], (i', j') = (i, j) for j <= N/2,
-// else the Hermitian mirror ((N-i)%N, N-j): the full-grid form of the half spectrum a c2r
-// transform reads.  F_a = (Z(k) + conj Z(-k)) / 2 and F_b = (Z(k) - conj Z(-k)) / 2i of the
+// G(i, j) = (Ga, Gb) with G(i, j) = Re F(i', j') * w[j'*N + i'], (i', j') = (i, j) for
+// j <= N/2, else the Hermitian mirror ((N-i)%N, N-j): the full-grid form of the half spectrum a
+// c2r transform reads.  F_a = (Z(k) + conj Z(-k)) / 2 and F_b = (Z(k) - conj Z(-k)) / 2i of the
 // packed transform Z, so Re F_a = (Re Z(k) + Re Z(-k)) / 2, Re F_b = (Im Z(k) + Im Z(-k)) / 2.
+// Both Z and G are held transposed (Z(a, b) at X[b*N + a], G(i, j) at Y[j*N + i]), the layout
+// the transforms leave and take without their final transpose; lanes run along i, so every
+// read (Z, its mirror, w[j'*N + i']) and the write are contiguous.
 template <typename R, typename C>
 __global__ void spec_mul(const C* __restrict__ X, const R* __restrict__ w, C* __restrict__ Y,
                          int N) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    const int i = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
     const int p = blockIdx.z;
-    if (j >= N) return;
+    if (i >= N) return;
     int ii = i, jj = j;
     if (j > N / 2) {
         ii = (N - i) % N;
         jj = N - j;
     }
     const size_t o = (size_t)p * N * N;
-    const C z = X[o + (size_t)ii * N + jj];
-    const C zm = X[o + (size_t)((N - ii) % N) * N + (N - jj) % N];
+    const C z = X[o + (size_t)jj * N + ii];
+    const C zm = X[o + (size_t)((N - jj) % N) * N + (N - ii) % N];
     R wk = w[(size_t)jj * N + ii];
     // columns 0 and N/2 hold both k and -k: the weight is averaged over the pair, so the
     // product stays even whatever the weights (a c2r transform keeps only that even part)
     if (jj == 0 || jj == N / 2) wk = (R)0.5 * (wk + w[(size_t)jj * N + (N - ii) % N]);
-    Y[o + (size_t)i * N + j] = C{(R)0.5 * (z.x + zm.x) * wk, (R)0.5 * (z.y + zm.y) * wk};
+    Y[o + (size_t)j * N + i] = C{(R)0.5 * (z.x + zm.x) * wk, (R)0.5 * (z.y + zm.y) * wk};
 }
 
 // dphi = exp(-0.5 (norm - D fact / fact2)), norm = D(N/2, N/2) fact / fact2, then fftshift;
@@ -148,15 +153,16 @@ __global__ void dphi_kernel(const C* __restrict__ Y, C* __restrict__ X, int N, d
                                  (R)exp(-0.5 * (nb - (double)d.y / fact2 * fact))};
 }
 
-// spec2: G[i][j] = otf[j'*N+i'] * dphi[j'*N+i'] with the same mirror for j > N/2, both
-// directions of the pair.
+// spec2: G(i, j) = otf[j'*N+i'] * dphi[j'*N+i'] with the same mirror for j > N/2, both
+// directions of the pair; dphi as dphi_kernel stores it, G written transposed (at Y[j*N + i])
+// with lanes along i, as in spec_mul.
 template <typename R, typename C>
 __global__ void spec_otf(const C* __restrict__ X, const R* __restrict__ otf, C* __restrict__ Y,
                          int N) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    const int i = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
     const int p = blockIdx.z;
-    if (j >= N) return;
+    if (i >= N) return;
     int ii = i, jj = j;
     if (j > N / 2) {
         ii = (N - i) % N;
@@ -168,11 +174,11 @@ __global__ void spec_otf(const C* __restrict__ X, const R* __restrict__ otf, C* 
     if (jj == 0 || jj == N / 2) {  // even part, as in spec_mul
         const size_t km = (size_t)jj * N + (N - ii) % N;
         const C dm = X[o + km];
-        Y[o + (size_t)i * N + j] = C{(R)0.5 * (otf[k] * d.x + otf[km] * dm.x),
+        Y[o + (size_t)j * N + i] = C{(R)0.5 * (otf[k] * d.x + otf[km] * dm.x),
                                      (R)0.5 * (otf[k] * d.y + otf[km] * dm.y)};
         return;
     }
-    Y[o + (size_t)i * N + j] = C{otf[k] * d.x, otf[k] * d.y};
+    Y[o + (size_t)j * N + i] = C{otf[k] * d.x, otf[k] * d.y};
 }
 
 template <typename C>
@@ -229,16 +235,19 @@ int process(moao_isample* S, const double* dCvv, int batch, double* dPsf, hipStr
                            S->Map, S->nmap, X, N, ncov, first, step, batch);
         MOAO_HIP_TRY(hipGetLastError());
     }
-    if (int rc = fft2d<R>(X, Y, N, pairs, -1, tw, s)) return rc;
+    // Transforms without their final transpose: a plain-layout input comes out transposed and
+    // a transposed one plain, so the spectra of steps 1 and 3 are held transposed and D and
+    // the PSF plain.
+    if (int rc = fft2d_t<R>(X, Y, N, pairs, -1, tw, s)) return rc;  // Y: Z transposed
     const dim3 pg((N + 255) / 256, N, pairs);
-    hipLaunchKernelGGL((spec_mul<R, C>), pg, dim3(256), 0, s, X, (const R*)S->a2f, Y, N);
-    if (int rc = fft2d<R>(Y, X, N, pairs, +1, tw, s)) return rc;
+    hipLaunchKernelGGL((spec_mul<R, C>), pg, dim3(256), 0, s, Y, (const R*)S->a2f, X, N);
+    if (int rc = fft2d_t<R>(X, Y, N, pairs, +1, tw, s)) return rc;  // Y: D
     const double fact = 2 * std::pow(2 * 3.14159 / S->lambda, 2);
     const double fact2 = (double)N * N * S->dactupix * S->dactupix;
     hipLaunchKernelGGL((dphi_kernel<R, C>), pg, dim3(256), 0, s, Y, X, N, fact, fact2);
     hipLaunchKernelGGL((spec_otf<R, C>), pg, dim3(256), 0, s, X, (const R*)S->otf, Y, N);
-    if (int rc = fft2d<R>(Y, X, N, pairs, +1, tw, s)) return rc;
-    hipLaunchKernelGGL(final_kernel<C>, pg, dim3(256), 0, s, Y, dPsf, N, batch);
+    if (int rc = fft2d_t<R>(Y, X, N, pairs, +1, tw, s)) return rc;  // X: the PSF before shift
+    hipLaunchKernelGGL(final_kernel<C>, pg, dim3(256), 0, s, X, dPsf, N, batch);
     MOAO_HIP_TRY(hipGetLastError());
     return MOAO_OK;
 }
